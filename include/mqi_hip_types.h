@@ -1,0 +1,47 @@
+/* Plain data types shared by the C ABI of libmqi_hip.so (include/mqi_hip.h)
+ * and its callers.  No torch or HIP types appear here. */
+#ifndef MQI_HIP_TYPES_H
+#define MQI_HIP_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One pencil-beam spot (a "beamlet" in the reference).  The phase-space and
+ * energy model is the reference's phsp_6d + norm_1d/const_1d beamlet
+ * (moqui/base/distributions/mqi_phsp6d.hpp:62-79, mqi_norm_1d.hpp:54-56,
+ * mqi_const_1d.hpp:55-57, mqi_beamlet.hpp:81-90): in the beam frame the
+ * position is mean + sigma * N(0,1) per axis, the direction x', y' are
+ * correlated with x, y through corr, and z' = -sqrt(1 - x'^2 - y'^2), so the
+ * beam travels along -z of its frame; the vertex is then mapped to the world
+ * by rotation * v + translation (coordinate_transform, mqi_coordinate_transform.hpp:27-122). */
+typedef struct mqi_spot {
+    float    energy;        /* mean kinetic energy, MeV */
+    float    sigma_energy;  /* Gaussian energy spread, MeV (ignored when energy_const) */
+    int32_t  energy_const;  /* 1: const_1d energy (no random draw), 0: norm_1d */
+    uint32_t reserved;
+    float    mean[6];       /* x, y, z (mm), x', y', z' in the beam frame */
+    float    sigma[6];      /* sigmas of the same; sigma[5] is ignored */
+    float    corr[2];       /* x-x' and y-y' correlation coefficients */
+    float    rotation[9];   /* beam frame -> world, row-major */
+    float    translation[3];/* mm */
+    uint64_t histories;     /* histories of this spot */
+} mqi_spot;
+
+/* Scorer bits.  DOSE: dose-to-water (Gy per history sum,
+ * mqi_scorer_energy_deposit.hpp:43-60); LETD: LETd numerator and
+ * denominator (:93-139); STAT: per-step dose and dose^2 for the
+ * statistical stopping criterion (mqi_tps_env.hpp:939-959). */
+enum {
+    MQI_SCORE_DOSE = 1,
+    MQI_SCORE_LETD = 2,
+    MQI_SCORE_STAT = 4
+};
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
