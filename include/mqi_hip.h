@@ -1,0 +1,183 @@
+/* libmqi_hip.so: MI355X (gfx950) transport and scoring for moqui's hot path.
+ *
+ * The C ABI replaces the reference's device boundary: the kernel
+ * mc::transport_particles_patient<R> / _stat<R>
+ * (moqui/kernel_functions/mqi_transport.hpp:113-126, :252-265), the marshalling
+ * functions mc::upload_node / upload_materials / upload_vertices /
+ * upload_scorer_offset_vector / download_node
+ * (moqui/kernel_functions/mqi_upload_data.hpp:148-450, mqi_download_data.hpp:37-121),
+ * the statistics kernels (moqui/kernel_functions/mqi_variables.hpp:21-66) and
+ * the run loops that drive them (moqui/base/environments/mqi_tps_env.hpp:1040-1603,
+ * mqi_phantom_env.hpp:303-428).
+ *
+ * Conventions: every call returns 0 on success or a nonzero MQI_E_* code and
+ * leaves a message for mqi_last_error(); nothing exits the process (the
+ * reference calls exit(), mqi_error_check.hpp:11-42).  A context owns one
+ * device and one stream; contexts are independent, so several may be used
+ * from different threads.  Host pointers are plain C arrays; the only device
+ * pointers in the ABI are the ones mqi_*_device_ptr hand out and the x / y
+ * vectors of mqi_dij_matvec.
+ */
+#ifndef MQI_HIP_H
+#define MQI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mqi_hip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mqi_ctx mqi_ctx;
+
+enum {
+    MQI_OK           = 0,
+    MQI_E_ARG        = 1,   /* invalid argument */
+    MQI_E_STATE      = 2,   /* call order: grid / spots / scorers missing */
+    MQI_E_HIP        = 3,   /* HIP runtime error */
+    MQI_E_MEMORY     = 4,   /* device allocation failed */
+    MQI_E_CAPACITY   = 5    /* a fixed-size table (Dij hash) overflowed */
+};
+
+/* scoring-loop semantics of a run */
+enum {
+    MQI_LOOPS_REFERENCE_PLAIN = 0, /* transport_particles_patient: scorers 0..n-3 scored twice when n >= 3
+                                      (mqi_transport.hpp:204-225) */
+    MQI_LOOPS_REFERENCE_STAT  = 1, /* transport_particles_patient_stat (mqi_transport.hpp:343-364) */
+    MQI_LOOPS_ONCE            = 2  /* each scorer once per step (default) */
+};
+
+typedef struct mqi_run_config {
+    uint64_t seed;             /* master seed of the per-history Philox streams */
+    int32_t  physics_debug;    /* 1: the __PHYSICS_DEBUG__ physics of the phantom build, 0: release */
+    int32_t  exact;            /* 1: verification kernels (portable math, no FMA), 0: production */
+    int32_t  batched;          /* 1: one persistent launch over all items; 0: one plain launch per spot */
+    int32_t  scoring_loops;    /* MQI_LOOPS_* */
+    int32_t  stack_depth;      /* secondary stack per lane; 0: the reference's (200 debug, 10 release) */
+    int32_t  block_size;       /* threads per workgroup; 0: 256 */
+    int32_t  blocks;           /* workgroups of the persistent launch; 0: fill the device */
+    int32_t  use_graph;        /* 1: capture the batched round in a hipGraph and replay it */
+    float    stat_criterion;   /* > 0: repeat rounds until mean(sd/mean) <= criterion (a fraction) */
+    float    stat_threshold;   /* voxels with mean > threshold * max mean enter the criterion */
+    int32_t  max_rounds;       /* cap on statistical rounds (0: 64) */
+    int32_t  dij;              /* 1: score the per-spot dose-influence tables */
+    uint64_t dij_capacity;     /* hash slots per (scenario, spot); 0: 131072 */
+    uint64_t hist_offset;      /* added to every history index (independent repeat runs) */
+} mqi_run_config;
+
+typedef struct mqi_run_result {
+    uint64_t histories;   /* histories transported (tracked_particles) */
+    uint64_t dropped;     /* secondaries dropped by full stacks (the reference drops them silently) */
+    uint32_t status;      /* bit 0: a Dij table overflowed; bit 1: a track hit the step cap (non-finite state) */
+    int32_t  rounds;      /* statistical rounds run */
+    double   kernel_ms;   /* device time of the transport launches (HIP events) */
+    double   stat;        /* last mean(sd/mean) as a fraction; -1 when not computed */
+} mqi_run_result;
+
+/* Context.  Replaces the process-global mc::mc_world / mc_vertices /
+ * mc_materials (mqi_variables.hpp:11-16) and the GPUID selection
+ * (mqi_tps_env.hpp:161-164). */
+int         mqi_create(int device, mqi_ctx** out);
+int         mqi_destroy(mqi_ctx* ctx);
+const char* mqi_last_error(const mqi_ctx* ctx);
+int         mqi_device_info(mqi_ctx* ctx, int* compute_units, size_t* total_memory);
+
+/* Voxel geometry of the patient/phantom child node: edges (n+1 per axis,
+ * ascending, mm) and the density in g/mm^3 with x fastest
+ * (grid3d::ijk2cnb, mqi_grid3d.hpp:403-413).  rot_fwd (row-major) and
+ * translation are grid3d::rotation_matrix_fwd and translation_vector; NULL
+ * means identity / zero.  Replaces add_node_geometry (mqi_upload_data.hpp:70-103). */
+int mqi_set_grid(mqi_ctx* ctx, int nx, int ny, int nz, const float* xe, const float* ye, const float* ze,
+                 const float* density, const float* rot_fwd, const float* translation);
+
+/* Same from a CT: density = hu_to_density(HU) * density_scale on the device
+ * (patient_material_t::hu_to_density, mqi_patient_materials.hpp:515-542, and
+ * the DensityScaling loop of tps_env::setup_world, mqi_tps_env.hpp:766-769). */
+int mqi_set_grid_hu(mqi_ctx* ctx, int nx, int ny, int nz, const float* xe, const float* ye, const float* ze,
+                    const int16_t* hu, float density_scale, const float* rot_fwd, const float* translation);
+
+/* Read back the density the device holds (nx*ny*nz floats). */
+int mqi_get_density(mqi_ctx* ctx, float* out);
+
+/* Spot table (the beamsource's beamlets with their history counts,
+ * mqi_beamsource.hpp:49-112).  Vertices are sampled on the device per history
+ * instead of on the host (phantom_env::setup_beamsource :205-232,
+ * tps_env::run_by_spot :1530-1552). */
+int mqi_set_spots(mqi_ctx* ctx, const mqi_spot* spots, int n_spots);
+
+/* Robust scenarios in one launch: per scenario an edge shift (mm, x y z) and
+ * a density scale (the XShift/YShift/ZShift and DensityScaling knobs,
+ * mqi_tps_env.hpp:475-477, 766-769).  n = 0 restores the single nominal scenario. */
+int mqi_set_scenarios(mqi_ctx* ctx, int n, const float* shift_xyz, const float* density_scale);
+
+/* Scorers (MQI_SCORE_* bits of mqi_hip_types.h), in the reference's order
+ * Dose, LETd numerator, LETd denominator, Dose_stat, DoseSquare_stat
+ * (mqi_tps_env.hpp:788-960); per_scenario != 0 keeps one array per scenario.
+ * Clears them.  Replaces add_node_scorers (mqi_upload_data.hpp:106-142). */
+int mqi_set_scorers(mqi_ctx* ctx, int mask, int per_scenario);
+int mqi_zero_scorers(mqi_ctx* ctx);
+int mqi_scorer_count(mqi_ctx* ctx);
+
+/* Transport all histories of all spots (and scenarios).  Replaces
+ * run_simulation / run_by_beam / run_by_beam_stat / run_by_spot
+ * (mqi_tps_env.hpp:1040-1603) and phantom_env::run (mqi_phantom_env.hpp:303-428). */
+int mqi_run(mqi_ctx* ctx, const mqi_run_config* cfg, mqi_run_result* res);
+
+/* Transport caller-made vertices (7 floats each: ke, x, y, z, u, v, w), the
+ * upload_vertices + transport_particles_patient path.  spot_ids may be NULL
+ * (the scorer_offset_vector, mqi_transport.hpp:150-154). */
+int mqi_run_vertices(mqi_ctx* ctx, const float* vertices, uint64_t n, const uint32_t* spot_ids,
+                     const mqi_run_config* cfg, mqi_run_result* res);
+
+/* Per-step record of the transport for verification: for history h, up to
+ * max_steps records of (voxel, its.dist, vtx0.pos, vtx0.dir, vtx1.pos, vtx0.ke, dE)
+ * at the point where the reference scores the step.  vertices (7 floats per
+ * history, may be NULL) replace the spot table as in mqi_run_vertices.
+ * cnb: n*max_steps, rec: n*max_steps*12, n_steps: n. */
+int mqi_trace(mqi_ctx* ctx, const mqi_run_config* cfg, const float* vertices, uint64_t n_histories, int max_steps,
+              int64_t* cnb, float* rec, int32_t* n_steps);
+
+/* Scorer k (0-based in the order above) of a scenario: copy to host
+ * (download_node, mqi_download_data.hpp:37-121) or hand out the device array. */
+int mqi_get_scorer(mqi_ctx* ctx, int k, int scenario, double* out);
+int mqi_scorer_device_ptr(mqi_ctx* ctx, int k, int scenario, double** dptr);
+
+/* mean(sd/mean) over voxels with mean > threshold * max mean from the Dose_stat
+ * and DoseSquare_stat scorers after n_histories histories
+ * (calculate_standard_deviation + calculate_stat, mqi_variables.hpp:21-48,
+ * mqi_tps_env.hpp:1340-1426). */
+int mqi_stat(mqi_ctx* ctx, int scenario, uint64_t n_histories, float threshold, double* stat);
+
+/* Dose-influence matrices.  mqi_dij_build turns the per-spot tables of the
+ * last run with cfg.dij = 1 into one CSR matrix per scenario, rows = spots,
+ * columns = voxels (the layout io::save_to_npz writes, mqi_io.hpp:249-320),
+ * float32 values and int32 indices, keeping entries >= rel_threshold times
+ * the row maximum; columns sorted; a voxel-major copy is kept for D w. */
+int mqi_dij_build(mqi_ctx* ctx, double rel_threshold);
+int mqi_dij_info(mqi_ctx* ctx, int scenario, int64_t* nnz, int64_t* rows, int64_t* cols);
+int mqi_dij_copy(mqi_ctx* ctx, int scenario, int64_t* indptr, int32_t* indices, float* values);
+int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, const int32_t** indices,
+                        const float** values);
+
+/* y = alpha * A x (+ y if accumulate) on the device, A = D (voxels x spots,
+ * transpose = 0, the forward dose D w) or D^T (transpose = 1, the adjoint
+ * D^T g).  x: float32 device vector; y: float64 if y_double else float32;
+ * FP64 accumulation.  stream: a hipStream_t or NULL for the context's stream. */
+int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
+                   int accumulate, void* stream);
+
+/* Device time (ms, HIP events) of the last mqi_dij_matvec, for measurement. */
+int mqi_last_matvec_ms(mqi_ctx* ctx, double* ms);
+
+/* Verification: the portable math functions on the device (fn 0 log, 1 exp,
+ * 2 pow, 3 sin, 4 cos, 5 acos, 6 sqrt, 7 float divide, 8 double divide,
+ * 9 float sqrt) on n host inputs. */
+int mqi_math_probe(mqi_ctx* ctx, int fn, const double* x, const double* y, int64_t n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,271 @@
+"""ctypes binding of libmqi_hip.so (include/mqi_hip.h).
+
+The library is loaded from this package directory; there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmqi_hip.so")
+
+SPOT_DTYPE = np.dtype([
+    ("energy", np.float32), ("sigma_energy", np.float32), ("energy_const", np.int32),
+    ("reserved", np.uint32), ("mean", np.float32, 6), ("sigma", np.float32, 6),
+    ("corr", np.float32, 2), ("rotation", np.float32, 9), ("translation", np.float32, 3),
+    ("histories", np.uint64)])
+assert SPOT_DTYPE.itemsize == 128
+
+SCORE_DOSE, SCORE_LETD, SCORE_STAT = 1, 2, 4
+LOOPS_REFERENCE_PLAIN, LOOPS_REFERENCE_STAT, LOOPS_ONCE = 0, 1, 2
+
+
+class RunConfig(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64), ("physics_debug", ctypes.c_int32), ("exact", ctypes.c_int32),
+        ("batched", ctypes.c_int32), ("scoring_loops", ctypes.c_int32), ("stack_depth", ctypes.c_int32),
+        ("block_size", ctypes.c_int32), ("blocks", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+        ("stat_criterion", ctypes.c_float), ("stat_threshold", ctypes.c_float), ("max_rounds", ctypes.c_int32),
+        ("dij", ctypes.c_int32), ("dij_capacity", ctypes.c_uint64), ("hist_offset", ctypes.c_uint64)]
+
+
+class RunResult(ctypes.Structure):
+    _fields_ = [
+        ("histories", ctypes.c_uint64), ("dropped", ctypes.c_uint64), ("status", ctypes.c_uint32),
+        ("rounds", ctypes.c_int32), ("kernel_ms", ctypes.c_double), ("stat", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmqi_hip.so is not built (python -m moquimc_amd.build): " + LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, u64, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+        sig = {
+            "mqi_create": (i, [i, ctypes.POINTER(vp)]),
+            "mqi_destroy": (i, [vp]),
+            "mqi_last_error": (ctypes.c_char_p, [vp]),
+            "mqi_device_info": (i, [vp, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_size_t)]),
+            "mqi_set_grid": (i, [vp, i, i, i, vp, vp, vp, vp, vp, vp]),
+            "mqi_set_grid_hu": (i, [vp, i, i, i, vp, vp, vp, vp, f, vp, vp]),
+            "mqi_get_density": (i, [vp, vp]),
+            "mqi_set_spots": (i, [vp, vp, i]),
+            "mqi_set_scenarios": (i, [vp, i, vp, vp]),
+            "mqi_set_scorers": (i, [vp, i, i]),
+            "mqi_zero_scorers": (i, [vp]),
+            "mqi_scorer_count": (i, [vp]),
+            "mqi_run": (i, [vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
+            "mqi_run_vertices": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
+            "mqi_trace": (i, [vp, ctypes.POINTER(RunConfig), vp, u64, i, vp, vp, vp]),
+            "mqi_get_scorer": (i, [vp, i, i, vp]),
+            "mqi_scorer_device_ptr": (i, [vp, i, i, ctypes.POINTER(vp)]),
+            "mqi_stat": (i, [vp, i, u64, f, ctypes.POINTER(d)]),
+            "mqi_dij_build": (i, [vp, d]),
+            "mqi_dij_info": (i, [vp, i, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_int64)]),
+            "mqi_dij_copy": (i, [vp, i, vp, vp, vp]),
+            "mqi_dij_device_ptrs": (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+            "mqi_dij_matvec": (i, [vp, i, i, vp, vp, i, d, i, vp]),
+            "mqi_last_matvec_ms": (i, [vp, ctypes.POINTER(d)]),
+            "mqi_math_probe": (i, [vp, i, vp, vp, ctypes.c_int64, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "mqi_set_grid", "mqi_set_grid_hu",
+            "mqi_get_density", "mqi_set_spots", "mqi_set_scenarios", "mqi_set_scorers", "mqi_zero_scorers",
+            "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
+            "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
+            "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe"]
+
+
+class MqiError(RuntimeError):
+    pass
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One device context of libmqi_hip.so (mqi_create / mqi_destroy)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.mqi_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise MqiError("mqi_create(%d) failed with code %d" % (device, rc))
+        self.h = h
+        self.nvox = 0
+        self.n_scen = 1
+        self.shape = None
+
+    def close(self):
+        if self.h:
+            self._L.mqi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self._L.mqi_last_error(self.h)
+            raise MqiError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def device_info(self):
+        cu = ctypes.c_int()
+        mem = ctypes.c_size_t()
+        self._chk(self._L.mqi_device_info(self.h, ctypes.byref(cu), ctypes.byref(mem)), "mqi_device_info")
+        return cu.value, mem.value
+
+    def set_grid(self, xe, ye, ze, density, rot=None, trans=None):
+        xe, ye, ze = (np.ascontiguousarray(a, np.float32) for a in (xe, ye, ze))
+        density = np.ascontiguousarray(density, np.float32).reshape(-1)
+        nx, ny, nz = len(xe) - 1, len(ye) - 1, len(ze) - 1
+        assert density.size == nx * ny * nz
+        rot = None if rot is None else np.ascontiguousarray(rot, np.float32)
+        trans = None if trans is None else np.ascontiguousarray(trans, np.float32)
+        self._chk(self._L.mqi_set_grid(self.h, nx, ny, nz, _p(xe), _p(ye), _p(ze), _p(density), _p(rot), _p(trans)),
+                  "mqi_set_grid")
+        self.nvox = nx * ny * nz
+        self.shape = (nz, ny, nx)
+
+    def set_grid_hu(self, xe, ye, ze, hu, density_scale=1.0, rot=None, trans=None):
+        xe, ye, ze = (np.ascontiguousarray(a, np.float32) for a in (xe, ye, ze))
+        hu = np.ascontiguousarray(hu, np.int16).reshape(-1)
+        nx, ny, nz = len(xe) - 1, len(ye) - 1, len(ze) - 1
+        assert hu.size == nx * ny * nz
+        rot = None if rot is None else np.ascontiguousarray(rot, np.float32)
+        trans = None if trans is None else np.ascontiguousarray(trans, np.float32)
+        self._chk(self._L.mqi_set_grid_hu(self.h, nx, ny, nz, _p(xe), _p(ye), _p(ze), _p(hu),
+                                          ctypes.c_float(density_scale), _p(rot), _p(trans)), "mqi_set_grid_hu")
+        self.nvox = nx * ny * nz
+        self.shape = (nz, ny, nx)
+
+    def density(self):
+        out = np.zeros(self.nvox, np.float32)
+        self._chk(self._L.mqi_get_density(self.h, _p(out)), "mqi_get_density")
+        return out
+
+    def set_spots(self, spots):
+        spots = np.ascontiguousarray(spots, SPOT_DTYPE)
+        self._chk(self._L.mqi_set_spots(self.h, _p(spots), len(spots)), "mqi_set_spots")
+        self.n_spots = len(spots)
+
+    def set_scenarios(self, shifts=None, scales=None):
+        if shifts is None:
+            self._chk(self._L.mqi_set_scenarios(self.h, 0, None, None), "mqi_set_scenarios")
+            self.n_scen = 1
+            return
+        shifts = np.ascontiguousarray(shifts, np.float32).reshape(-1, 3)
+        scales = np.ascontiguousarray(scales, np.float32).reshape(-1)
+        assert len(shifts) == len(scales)
+        self._chk(self._L.mqi_set_scenarios(self.h, len(scales), _p(shifts), _p(scales)), "mqi_set_scenarios")
+        self.n_scen = len(scales)
+
+    def set_scorers(self, mask, per_scenario=False):
+        self._chk(self._L.mqi_set_scorers(self.h, mask, 1 if per_scenario else 0), "mqi_set_scorers")
+
+    def zero_scorers(self):
+        self._chk(self._L.mqi_zero_scorers(self.h), "mqi_zero_scorers")
+
+    def scorer_count(self):
+        return self._L.mqi_scorer_count(self.h)
+
+    @staticmethod
+    def config(seed=1, physics_debug=False, exact=False, batched=True, loops=LOOPS_ONCE, stack_depth=0,
+               block_size=0, blocks=0, use_graph=False, stat_criterion=0.0, stat_threshold=0.5, max_rounds=0,
+               dij=False, dij_capacity=0, hist_offset=0):
+        return RunConfig(seed, int(physics_debug), int(exact), int(batched), loops, stack_depth, block_size, blocks,
+                         int(use_graph), stat_criterion, stat_threshold, max_rounds, int(dij), dij_capacity,
+                         hist_offset)
+
+    def run(self, cfg):
+        res = RunResult()
+        self._chk(self._L.mqi_run(self.h, ctypes.byref(cfg), ctypes.byref(res)), "mqi_run")
+        return res
+
+    def run_vertices(self, vertices, cfg, spot_ids=None):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 7)
+        sid = None if spot_ids is None else np.ascontiguousarray(spot_ids, np.uint32)
+        res = RunResult()
+        self._chk(self._L.mqi_run_vertices(self.h, _p(v), len(v), _p(sid), ctypes.byref(cfg), ctypes.byref(res)),
+                  "mqi_run_vertices")
+        return res
+
+    def trace(self, cfg, n_histories, max_steps, vertices=None):
+        v = None if vertices is None else np.ascontiguousarray(vertices, np.float32).reshape(-1, 7)
+        if v is not None:
+            n_histories = len(v)
+        cnb = np.zeros((n_histories, max_steps), np.int64)
+        rec = np.zeros((n_histories, max_steps, 12), np.float32)
+        nst = np.zeros(n_histories, np.int32)
+        self._chk(self._L.mqi_trace(self.h, ctypes.byref(cfg), _p(v), n_histories, max_steps, _p(cnb), _p(rec),
+                                    _p(nst)), "mqi_trace")
+        return cnb, rec, nst
+
+    def scorer(self, k, scenario=0):
+        out = np.zeros(self.nvox, np.float64)
+        self._chk(self._L.mqi_get_scorer(self.h, k, scenario, _p(out)), "mqi_get_scorer")
+        return out
+
+    def scorer_device_ptr(self, k, scenario=0):
+        p = ctypes.c_void_p()
+        self._chk(self._L.mqi_scorer_device_ptr(self.h, k, scenario, ctypes.byref(p)), "mqi_scorer_device_ptr")
+        return p.value
+
+    def stat(self, n_histories, threshold=0.5, scenario=0):
+        out = ctypes.c_double()
+        self._chk(self._L.mqi_stat(self.h, scenario, n_histories, ctypes.c_float(threshold), ctypes.byref(out)),
+                  "mqi_stat")
+        return out.value
+
+    def dij_build(self, rel_threshold=1e-4):
+        self._chk(self._L.mqi_dij_build(self.h, ctypes.c_double(rel_threshold)), "mqi_dij_build")
+
+    def dij_info(self, scenario=0):
+        nnz, rows, cols = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self._L.mqi_dij_info(self.h, scenario, ctypes.byref(nnz), ctypes.byref(rows), ctypes.byref(cols)),
+                  "mqi_dij_info")
+        return nnz.value, rows.value, cols.value
+
+    def dij_csr(self, scenario=0):
+        nnz, rows, cols = self.dij_info(scenario)
+        indptr = np.zeros(rows + 1, np.int64)
+        idx = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1), np.float32)
+        self._chk(self._L.mqi_dij_copy(self.h, scenario, _p(indptr), _p(idx), _p(val)), "mqi_dij_copy")
+        return indptr, idx[:nnz], val[:nnz], (rows, cols)
+
+    def dij_matvec(self, scenario, transpose, x_ptr, y_ptr, y_double=True, alpha=1.0, accumulate=False, stream=None):
+        self._chk(self._L.mqi_dij_matvec(self.h, scenario, int(transpose), ctypes.c_void_p(x_ptr),
+                                         ctypes.c_void_p(y_ptr), int(y_double), ctypes.c_double(alpha),
+                                         int(accumulate), None if stream is None else ctypes.c_void_p(stream)),
+                  "mqi_dij_matvec")
+
+    def last_matvec_ms(self):
+        ms = ctypes.c_double()
+        self._chk(self._L.mqi_last_matvec_ms(self.h, ctypes.byref(ms)), "mqi_last_matvec_ms")
+        return ms.value
+
+    def math_probe(self, fn, x, y=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = x if y is None else np.ascontiguousarray(y, np.float64)
+        out = np.zeros_like(x)
+        self._chk(self._L.mqi_math_probe(self.h, fn, _p(x), _p(y), len(x), _p(out)), "mqi_math_probe")
+        return out

@@ -1,0 +1,1026 @@
+// Host runtime behind the C ABI of include/mqi_hip.h.
+//
+// A context holds the device copies of one voxel child (edges, density), the
+// spot table, the scenario table, the scorer arrays (FP64, dense, one slot
+// per voxel instead of the reference's key_value tables), the per-lane
+// secondary stacks, the Dij hash tables and the CSR matrices built from them.
+// All device work is enqueued on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mqi_hip.h"
+#include "mqi_aux.h"
+#include "mqi_kernels.h"
+
+namespace
+{
+
+struct csr_dev {
+    int64_t  rows = 0, cols = 0, nnz = 0;
+    int64_t* indptr  = nullptr;   // rows + 1
+    int32_t* col     = nullptr;
+    float*   val     = nullptr;
+    int64_t* tindptr = nullptr;   // transposed (voxel-major) copy: cols + 1
+    int32_t* tcol    = nullptr;
+    float*   tval    = nullptr;
+};
+
+}   // namespace
+
+struct mqi_ctx {
+    int         device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int         n_cu   = 0;
+    size_t      memory = 0;
+
+    // geometry
+    int                nx = 0, ny = 0, nz = 0;
+    float *            d_xe = nullptr, *d_ye = nullptr, *d_ze = nullptr, *d_density = nullptr;
+    float              fwd[9], inv[9], trans[3];
+    std::vector<float> h_xe, h_ye, h_ze;
+
+    // spots and scenarios
+    std::vector<mqi_spot> spots;
+    mqi_spot*             d_spots = nullptr;
+    int                   n_scen  = 1;
+    float *               d_shift = nullptr, *d_scale = nullptr;
+
+    // scorers
+    int              mask = 0, per_scen = 0;
+    std::vector<int> kinds;
+    double*          d_out[MQI_MAX_SCORERS] = { nullptr };
+
+    // counters: next, tracked, drops
+    unsigned long long* d_counters = nullptr;
+    unsigned int*       d_status   = nullptr;
+    uint64_t*           d_hoff     = nullptr;
+    float*              d_statmax  = nullptr;
+    double*             d_statsum  = nullptr;
+    unsigned long long* d_statcnt  = nullptr;
+
+    // per-lane secondary stacks
+    float* d_stack     = nullptr;
+    size_t stack_bytes = 0;
+
+    // work list
+    uint64_t* d_item_end  = nullptr;
+    uint32_t* d_item_spot = nullptr;
+    uint32_t* d_item_scen = nullptr;
+    size_t    items_cap   = 0;
+
+    // Dij
+    uint32_t*            d_dij_keys = nullptr;
+    double*              d_dij_vals = nullptr;
+    uint64_t             dij_cap    = 0;
+    size_t               dij_items  = 0;
+    std::vector<csr_dev> dij;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double     last_matvec_ms = 0.0;
+};
+
+namespace
+{
+
+int
+fail(mqi_ctx* c, int code, const char* fmt, ...) {
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                                    \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return fail(c, MQI_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+#define KCHK(c, expr)                                                                                      \
+    do {                                                                                                   \
+        int e_ = (expr);                                                                                   \
+        if (e_ != 0) return fail(c, MQI_E_HIP, "%s: %s", #expr, hipGetErrorString((hipError_t) e_));      \
+    } while (0)
+
+template<typename T>
+int
+dalloc(mqi_ctx* c, T** p, size_t count) {
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) return 0;
+    hipError_t e = hipMalloc((void**) p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, MQI_E_MEMORY, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return 0;
+}
+
+template<typename T>
+void
+dfree(T*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+void
+free_dij(mqi_ctx* c) {
+    for (auto& m : c->dij) {
+        dfree(m.indptr);
+        dfree(m.col);
+        dfree(m.val);
+        dfree(m.tindptr);
+        dfree(m.tcol);
+        dfree(m.tval);
+    }
+    c->dij.clear();
+}
+
+uint64_t
+nvox(const mqi_ctx* c) {
+    return (uint64_t) c->nx * c->ny * c->nz;
+}
+
+int
+check_edges(mqi_ctx* c, const float* e, int n, const char* axis) {
+    if (n < 1) return fail(c, MQI_E_ARG, "%s: need at least one voxel", axis);
+    for (int i = 0; i < n; ++i) {
+        if (!(e[i + 1] > e[i])) return fail(c, MQI_E_ARG, "%s edges must ascend (index %d)", axis, i);
+        // The device index search tests only the three edges around the
+        // position; that reproduces the reference's linear scan when every
+        // voxel is wider than twice the 1e-3 mm tolerance.
+        if (!((double) e[i + 1] - (double) e[i] > 2.5e-3))
+            return fail(c, MQI_E_ARG, "%s voxel %d narrower than 2.5e-3 mm", axis, i);
+    }
+    return 0;
+}
+
+int
+upload_grid_common(mqi_ctx* c, int nx, int ny, int nz, const float* xe, const float* ye, const float* ze,
+                   const float* rot, const float* tr) {
+    if (!xe || !ye || !ze) return fail(c, MQI_E_ARG, "edges are NULL");
+    int r;
+    if ((r = check_edges(c, xe, nx, "x")) || (r = check_edges(c, ye, ny, "y")) || (r = check_edges(c, ze, nz, "z")))
+        return r;
+    if ((uint64_t) nx * ny * nz >= (1ull << 31)) return fail(c, MQI_E_ARG, "more than 2^31 voxels");
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->h_xe.assign(xe, xe + nx + 1);
+    c->h_ye.assign(ye, ye + ny + 1);
+    c->h_ze.assign(ze, ze + nz + 1);
+    if ((r = dalloc(c, &c->d_xe, nx + 1)) || (r = dalloc(c, &c->d_ye, ny + 1)) || (r = dalloc(c, &c->d_ze, nz + 1)))
+        return r;
+    HIPCHK(c, hipMemcpyAsync(c->d_xe, xe, sizeof(float) * (nx + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ye, ye, sizeof(float) * (ny + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_ze, ze, sizeof(float) * (nz + 1), hipMemcpyHostToDevice, c->stream));
+    const float I[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
+    for (int k = 0; k < 9; ++k) c->fwd[k] = rot ? rot[k] : I[k];
+    // grid3d::rotation_matrix_inv = fwd.inverse() (the transpose, mqi_matrix.hpp:290-293)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) c->inv[3 * i + j] = c->fwd[3 * j + i];
+    for (int k = 0; k < 3; ++k) c->trans[k] = tr ? tr[k] : 0.0f;
+    return dalloc(c, &c->d_density, nvox(c));
+}
+
+size_t
+shmem_bytes(const mqi_ctx* c) {
+    return sizeof(float) * (3600 + (size_t) c->nx + 1 + c->ny + 1 + c->nz + 1);
+}
+
+int
+ensure_items(mqi_ctx* c, size_t n) {
+    if (n <= c->items_cap) return 0;
+    int r;
+    if ((r = dalloc(c, &c->d_item_end, n)) || (r = dalloc(c, &c->d_item_spot, n)) || (r = dalloc(c, &c->d_item_scen, n)))
+        return r;
+    c->items_cap = n;
+    return 0;
+}
+
+int
+ensure_stack(mqi_ctx* c, size_t bytes) {
+    if (bytes <= c->stack_bytes) return 0;
+    dfree(c->d_stack);
+    int r = dalloc(c, (char**) &c->d_stack, bytes);
+    if (r) return r;
+    c->stack_bytes = bytes;
+    return 0;
+}
+
+struct run_plan {
+    std::vector<uint64_t> item_end;
+    std::vector<uint32_t> item_spot, item_scen;
+    uint64_t              total    = 0;
+    uint64_t              max_item = 0;
+};
+
+// items are (scenario, spot), scenario-major; empty spots are skipped
+run_plan
+make_plan(const mqi_ctx* c, int only_spot) {
+    run_plan p;
+    for (int s = 0; s < c->n_scen; ++s)
+        for (size_t j = 0; j < c->spots.size(); ++j) {
+            if (only_spot >= 0 && (int) j != only_spot) continue;
+            uint64_t h = c->spots[j].histories;
+            if (h == 0) continue;
+            p.total += h;
+            p.max_item = std::max(p.max_item, h);
+            p.item_end.push_back(p.total);
+            p.item_spot.push_back((uint32_t) j);
+            p.item_scen.push_back((uint32_t) s);
+        }
+    return p;
+}
+
+void
+fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
+    std::memset(&P, 0, sizeof P);
+    P.nx      = c->nx;
+    P.ny      = c->ny;
+    P.nz      = c->nz;
+    P.xe      = c->d_xe;
+    P.ye      = c->d_ye;
+    P.ze      = c->d_ze;
+    P.density = c->d_density;
+    std::memcpy(P.fwd, c->fwd, sizeof P.fwd);
+    std::memcpy(P.inv, c->inv, sizeof P.inv);
+    std::memcpy(P.trans, c->trans, sizeof P.trans);
+    P.scen_shift    = c->n_scen > 1 ? c->d_shift : nullptr;
+    P.scen_scale    = c->n_scen > 1 ? c->d_scale : nullptr;
+    P.spots         = c->d_spots;
+    P.n_spots       = (int) c->spots.size();
+    P.item_end      = c->d_item_end;
+    P.item_spot     = c->d_item_spot;
+    P.item_scenario = c->d_item_scen;
+    P.next          = c->d_counters;
+    P.tracked       = c->d_counters + 1;
+    P.drops         = c->d_counters + 2;
+    P.status        = c->d_status;
+    P.hist_offset   = c->d_hoff;
+    P.seed          = cfg.seed;
+    P.n_scorers     = (int) c->kinds.size();
+    for (size_t k = 0; k < c->kinds.size(); ++k) {
+        P.kinds[k] = c->kinds[k];
+        P.out[k]   = c->d_out[k];
+    }
+    if (cfg.dij) {
+        P.kinds[P.n_scorers] = MQI_KIND_DIJ;
+        P.out[P.n_scorers]   = nullptr;
+        P.n_scorers++;
+        P.dij_keys     = c->d_dij_keys;
+        P.dij_vals     = c->d_dij_vals;
+        P.dij_capacity = c->dij_cap;
+    }
+    P.out_scenario_stride = c->per_scen ? nvox(c) : 0;
+    P.loops               = cfg.scoring_loops;
+    P.stack               = c->d_stack;
+    P.stack_cap           = cfg.stack_depth > 0 ? cfg.stack_depth : (cfg.physics_debug ? 200 : 10);
+}
+
+// Every device array a launch reads must be live: the work list and the
+// stack can be reallocated between fill_params and the launch, so the
+// pointers are bound here, and the launch refuses NULL ones.
+void
+bind_buffers(mqi_ctx* c, transport_params& P) {
+    P.item_end      = c->d_item_end;
+    P.item_spot     = c->d_item_spot;
+    P.item_scenario = c->d_item_scen;
+    P.stack         = c->d_stack;
+}
+
+int
+launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
+       int threads) {
+    bind_buffers(c, P);
+    if (!P.density || !P.xe || !P.ye || !P.ze || !P.stack || !P.next || !P.tracked || !P.drops || !P.status ||
+        !P.hist_offset || (!P.vertices && (!P.spots || !P.item_end || !P.item_spot || !P.item_scenario)))
+        return (int) hipErrorInvalidValue;
+    for (int s = 0; s < P.n_scorers; ++s)
+        if (P.kinds[s] == MQI_KIND_DIJ ? (!P.dij_keys || !P.dij_vals) : !P.out[s]) return (int) hipErrorInvalidValue;
+    if (trace && (!P.trace_cnb || !P.trace_rec || !P.trace_n)) return (int) hipErrorInvalidValue;
+    if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * 9 * sizeof(float) > c->stack_bytes)
+        return (int) hipErrorInvalidValue;
+    const size_t sh = shmem_bytes(c);
+    if (cfg.exact) return mqi_exact::launch_transport(P, cfg.physics_debug, dynamic, trace, blocks, threads, sh, c->stream);
+    return mqi_fast::launch_transport(P, cfg.physics_debug, dynamic, trace, blocks, threads, sh, c->stream);
+}
+
+int
+persistent_blocks(mqi_ctx* c, const mqi_run_config& cfg, int threads, int trace) {
+    if (cfg.blocks > 0) return cfg.blocks;
+    const size_t sh  = shmem_bytes(c);
+    int          occ = cfg.exact ? mqi_exact::occupancy(cfg.physics_debug, 1, trace, threads, sh)
+                                 : mqi_fast::occupancy(cfg.physics_debug, 1, trace, threads, sh);
+    return c->n_cu * occ;
+}
+
+int
+upload_plan(mqi_ctx* c, const run_plan& p) {
+    int r = ensure_items(c, std::max<size_t>(1, p.item_end.size()));
+    if (r) return r;
+    if (p.item_end.empty()) return 0;
+    HIPCHK(c, hipMemcpyAsync(c->d_item_end, p.item_end.data(), sizeof(uint64_t) * p.item_end.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_item_spot, p.item_spot.data(), sizeof(uint32_t) * p.item_spot.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_item_scen, p.item_scen.data(), sizeof(uint32_t) * p.item_scen.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// One pass over a plan: the persistent dynamic launch, or (plain) one static
+// launch per spot of the plan.  Adds the device time of the transport
+// kernels to *ms.
+int
+transport_pass(mqi_ctx* c, const mqi_run_config& cfg, uint64_t hoff, double* ms) {
+    const int threads = cfg.block_size > 0 ? cfg.block_size : 256;
+    if (threads % 64 != 0 || threads > 256) return fail(c, MQI_E_ARG, "block_size must be 64, 128, 192 or 256");
+    transport_params P;
+    fill_params(c, P, cfg);
+    KCHK(c, mqi_aux::fill_u64(c->d_hoff, hoff, 1, c->stream));
+    const size_t lane_bytes = (size_t) P.stack_cap * 9 * sizeof(float);
+    int          r;
+    if (cfg.batched) {
+        run_plan p = make_plan(c, -1);
+        if ((r = upload_plan(c, p))) return r;
+        if (p.total == 0) return 0;
+        P.n_items      = (int) p.item_end.size();
+        P.total        = p.total;
+        const int blocks = persistent_blocks(c, cfg, threads, 0);
+        if ((r = ensure_stack(c, lane_bytes * (size_t) blocks * threads))) return r;
+        P.stack = c->d_stack;
+        KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 1, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        KCHK(c, launch(c, P, cfg, 1, 0, blocks, threads));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float t = 0;
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+        *ms += t;
+        return 0;
+    }
+    // plain: per scenario, per spot
+    const uint64_t max_lanes = 1ull << 18;
+    const int      max_blocks = (int) ((max_lanes + threads - 1) / threads);
+    if ((r = ensure_stack(c, lane_bytes * (size_t) max_blocks * threads))) return r;
+    P.stack = c->d_stack;
+    for (size_t j = 0; j < c->spots.size(); ++j) {
+        run_plan p = make_plan(c, (int) j);
+        if (p.total == 0) continue;
+        if ((r = upload_plan(c, p))) return r;
+        P.n_items    = (int) p.item_end.size();
+        P.total      = p.total;
+        uint64_t lanes = std::min<uint64_t>(p.total, max_lanes);
+        int      blocks = (int) ((lanes + threads - 1) / threads);   // <= max_blocks
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        KCHK(c, launch(c, P, cfg, 0, 0, blocks, threads));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float t = 0;
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+        *ms += t;
+    }
+    return 0;
+}
+
+int
+stat_index(const mqi_ctx* c, int* s1, int* s2) {
+    *s1 = *s2 = -1;
+    for (size_t k = 0; k < c->kinds.size(); ++k) {
+        if (c->kinds[k] == MQI_KIND_STAT_DOSE) *s1 = (int) k;
+        if (c->kinds[k] == MQI_KIND_STAT_DOSE2) *s2 = (int) k;
+    }
+    return (*s1 >= 0 && *s2 >= 0) ? 0 : -1;
+}
+
+int
+stat_value(mqi_ctx* c, int scenario, uint64_t n, float thr, double* out) {
+    int s1, s2;
+    if (stat_index(c, &s1, &s2)) return fail(c, MQI_E_STATE, "statistics need the MQI_SCORE_STAT scorers");
+    if (n < 2) return fail(c, MQI_E_ARG, "statistics need at least two histories");
+    const uint64_t off = c->per_scen ? (uint64_t) scenario * nvox(c) : 0;
+    KCHK(c, mqi_aux::stat_reduce(c->d_out[s1] + off, c->d_out[s2] + off, nvox(c), (double) n, thr, c->d_statmax,
+                                 c->d_statsum, c->d_statcnt, c->stream));
+    double             sum = 0;
+    unsigned long long cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&sum, c->d_statsum, sizeof sum, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&cnt, c->d_statcnt, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = cnt ? sum / (double) cnt : 0.0;
+    return 0;
+}
+
+int
+ready(mqi_ctx* c) {
+    if (!c) return MQI_E_ARG;
+    if (!c->d_density) return fail(c, MQI_E_STATE, "no grid: call mqi_set_grid first");
+    return 0;
+}
+
+int
+alloc_dij(mqi_ctx* c, const mqi_run_config& cfg) {
+    const uint64_t cap   = cfg.dij_capacity ? cfg.dij_capacity : 131072;
+    const size_t   items = (size_t) c->n_scen * c->spots.size();
+    int            r;
+    if (cap != c->dij_cap || items != c->dij_items) {
+        if ((r = dalloc(c, &c->d_dij_keys, cap * items)) || (r = dalloc(c, &c->d_dij_vals, cap * items))) return r;
+        c->dij_cap   = cap;
+        c->dij_items = items;
+    }
+    // items are indexed scenario * n_spots + spot (every spot, including empty ones)
+    KCHK(c, mqi_aux::fill_u32(c->d_dij_keys, 0xffffffffu, cap * items, c->stream));
+    KCHK(c, mqi_aux::fill_u64(c->d_dij_vals, 0ull, cap * items, c->stream));
+    return 0;
+}
+
+}   // namespace
+
+extern "C" {
+
+int
+mqi_create(int device, mqi_ctx** out) {
+    if (!out) return MQI_E_ARG;
+    *out   = nullptr;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return MQI_E_HIP;
+    if (device < 0 || device >= nd) return MQI_E_ARG;
+    mqi_ctx* c = new (std::nothrow) mqi_ctx();
+    if (!c) return MQI_E_MEMORY;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return MQI_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->n_cu   = prop.multiProcessorCount;
+        c->memory = prop.totalGlobalMem;
+    }
+    if (c->n_cu <= 0) c->n_cu = 1;
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    int r;
+    if ((r = dalloc(c, &c->d_counters, 4)) || (r = dalloc(c, &c->d_status, 1)) || (r = dalloc(c, &c->d_hoff, 1)) ||
+        (r = dalloc(c, &c->d_statmax, 1)) || (r = dalloc(c, &c->d_statsum, 1)) || (r = dalloc(c, &c->d_statcnt, 1))) {
+        mqi_destroy(c);
+        return r;
+    }
+    const float one = 1.0f, zero3[3] = { 0, 0, 0 };
+    if ((r = dalloc(c, &c->d_shift, 3)) || (r = dalloc(c, &c->d_scale, 1))) {
+        mqi_destroy(c);
+        return r;
+    }
+    hipMemcpy(c->d_shift, zero3, sizeof zero3, hipMemcpyHostToDevice);
+    hipMemcpy(c->d_scale, &one, sizeof one, hipMemcpyHostToDevice);
+    *out = c;
+    return 0;
+}
+
+int
+mqi_destroy(mqi_ctx* c) {
+    if (!c) return MQI_E_ARG;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    dfree(c->d_xe);
+    dfree(c->d_ye);
+    dfree(c->d_ze);
+    dfree(c->d_density);
+    dfree(c->d_spots);
+    dfree(c->d_shift);
+    dfree(c->d_scale);
+    for (auto& p : c->d_out) dfree(p);
+    dfree(c->d_counters);
+    dfree(c->d_status);
+    dfree(c->d_hoff);
+    dfree(c->d_statmax);
+    dfree(c->d_statsum);
+    dfree(c->d_statcnt);
+    dfree(c->d_stack);
+    dfree(c->d_item_end);
+    dfree(c->d_item_spot);
+    dfree(c->d_item_scen);
+    dfree(c->d_dij_keys);
+    dfree(c->d_dij_vals);
+    free_dij(c);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+const char*
+mqi_last_error(const mqi_ctx* c) {
+    return c ? c->err.c_str() : "no context";
+}
+
+int
+mqi_device_info(mqi_ctx* c, int* cus, size_t* mem) {
+    if (!c) return MQI_E_ARG;
+    if (cus) *cus = c->n_cu;
+    if (mem) *mem = c->memory;
+    return 0;
+}
+
+int
+mqi_set_grid(mqi_ctx* c, int nx, int ny, int nz, const float* xe, const float* ye, const float* ze,
+             const float* density, const float* rot, const float* tr) {
+    if (!c || !density) return fail(c, MQI_E_ARG, "density is NULL");
+    hipSetDevice(c->device);
+    int r = upload_grid_common(c, nx, ny, nz, xe, ye, ze, rot, tr);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(c->d_density, density, sizeof(float) * nvox(c), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->mask) return mqi_set_scorers(c, c->mask, c->per_scen);
+    return 0;
+}
+
+int
+mqi_set_grid_hu(mqi_ctx* c, int nx, int ny, int nz, const float* xe, const float* ye, const float* ze,
+                const int16_t* hu, float density_scale, const float* rot, const float* tr) {
+    if (!c || !hu) return fail(c, MQI_E_ARG, "hu is NULL");
+    hipSetDevice(c->device);
+    int r = upload_grid_common(c, nx, ny, nz, xe, ye, ze, rot, tr);
+    if (r) return r;
+    int16_t* d_hu = nullptr;
+    if ((r = dalloc(c, &d_hu, nvox(c)))) return r;
+    HIPCHK(c, hipMemcpyAsync(d_hu, hu, sizeof(int16_t) * nvox(c), hipMemcpyHostToDevice, c->stream));
+    KCHK(c, mqi_aux::hu_to_density(d_hu, c->d_density, nvox(c), density_scale, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipFree(d_hu);
+    if (c->mask) return mqi_set_scorers(c, c->mask, c->per_scen);
+    return 0;
+}
+
+int
+mqi_get_density(mqi_ctx* c, float* out) {
+    int r = ready(c);
+    if (r) return r;
+    HIPCHK(c, hipMemcpy(out, c->d_density, sizeof(float) * nvox(c), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int
+mqi_set_spots(mqi_ctx* c, const mqi_spot* spots, int n) {
+    if (!c || n < 0 || (n > 0 && !spots)) return fail(c, MQI_E_ARG, "bad spot table");
+    hipSetDevice(c->device);
+    for (int i = 0; i < n; ++i) {
+        if (!(spots[i].energy > 0.5f) || !(spots[i].energy < 299.0f))
+            return fail(c, MQI_E_ARG, "spot %d: energy %g MeV outside the tables (0.5, 299)", i, spots[i].energy);
+        if (spots[i].histories >= (1ull << 32))
+            return fail(c, MQI_E_ARG, "spot %d: more than 2^32 histories", i);
+    }
+    c->spots.assign(spots, spots + n);
+    int r = dalloc(c, &c->d_spots, std::max(1, n));
+    if (r) return r;
+    if (n) HIPCHK(c, hipMemcpy(c->d_spots, spots, sizeof(mqi_spot) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int
+mqi_set_scenarios(mqi_ctx* c, int n, const float* shift, const float* scale) {
+    if (!c || n < 0) return MQI_E_ARG;
+    hipSetDevice(c->device);
+    if (n == 0) {
+        c->n_scen = 1;
+    } else {
+        if (!shift || !scale) return fail(c, MQI_E_ARG, "scenario arrays are NULL");
+        int r;
+        if ((r = dalloc(c, &c->d_shift, 3 * (size_t) n)) || (r = dalloc(c, &c->d_scale, (size_t) n))) return r;
+        HIPCHK(c, hipMemcpy(c->d_shift, shift, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_scale, scale, sizeof(float) * n, hipMemcpyHostToDevice));
+        c->n_scen = n;
+    }
+    if (c->mask && c->per_scen) return mqi_set_scorers(c, c->mask, c->per_scen);
+    return 0;
+}
+
+int
+mqi_set_scorers(mqi_ctx* c, int mask, int per_scenario) {
+    int r = ready(c);
+    if (r) return r;
+    if (mask & ~(MQI_SCORE_DOSE | MQI_SCORE_LETD | MQI_SCORE_STAT)) return fail(c, MQI_E_ARG, "unknown scorer bits");
+    c->mask     = mask;
+    c->per_scen = per_scenario ? 1 : 0;
+    c->kinds.clear();
+    if (mask & MQI_SCORE_DOSE) c->kinds.push_back(MQI_KIND_DOSE);
+    if (mask & MQI_SCORE_LETD) {
+        c->kinds.push_back(MQI_KIND_LETD_NUM);
+        c->kinds.push_back(MQI_KIND_LETD_DEN);
+    }
+    if (mask & MQI_SCORE_STAT) {
+        c->kinds.push_back(MQI_KIND_STAT_DOSE);
+        c->kinds.push_back(MQI_KIND_STAT_DOSE2);
+    }
+    const uint64_t n = nvox(c) * (c->per_scen ? (uint64_t) c->n_scen : 1ull);
+    for (int k = 0; k < MQI_MAX_SCORERS; ++k) {
+        if (k < (int) c->kinds.size()) {
+            if ((r = dalloc(c, &c->d_out[k], n))) return r;
+        } else {
+            dfree(c->d_out[k]);
+        }
+    }
+    return mqi_zero_scorers(c);
+}
+
+int
+mqi_zero_scorers(mqi_ctx* c) {
+    int r = ready(c);
+    if (r) return r;
+    const uint64_t n = nvox(c) * (c->per_scen ? (uint64_t) c->n_scen : 1ull);
+    for (size_t k = 0; k < c->kinds.size(); ++k) KCHK(c, mqi_aux::fill_u64(c->d_out[k], 0ull, n, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int
+mqi_scorer_count(mqi_ctx* c) {
+    return c ? (int) c->kinds.size() : 0;
+}
+
+int
+mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
+    int r = ready(c);
+    if (r) return r;
+    if (!cfg_in) return fail(c, MQI_E_ARG, "cfg is NULL");
+    if (c->spots.empty()) return fail(c, MQI_E_STATE, "no spots: call mqi_set_spots first");
+    hipSetDevice(c->device);
+    mqi_run_config cfg = *cfg_in;
+    if (cfg.scoring_loops < 0 || cfg.scoring_loops > 2) return fail(c, MQI_E_ARG, "bad scoring_loops");
+    const int n_sc = (int) c->kinds.size() + (cfg.dij ? 1 : 0);
+    if (n_sc == 0) return fail(c, MQI_E_STATE, "no scorers");
+    if (cfg.dij && (r = alloc_dij(c, cfg))) return r;
+    KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 4, c->stream));
+    KCHK(c, mqi_aux::fill_u32(c->d_status, 0u, 1, c->stream));
+    mqi_run_result out;
+    std::memset(&out, 0, sizeof out);
+    out.stat          = -1.0;
+    const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 64;
+    run_plan  p          = make_plan(c, -1);
+    uint64_t  round_len  = p.max_item;
+    double    ms         = 0;
+    int       rounds     = 0;
+    for (;;) {
+        if ((r = transport_pass(c, cfg, cfg.hist_offset + (uint64_t) rounds * round_len, &ms))) return r;
+        ++rounds;
+        if (!(cfg.stat_criterion > 0)) break;
+        unsigned long long tracked = 0;
+        HIPCHK(c, hipMemcpy(&tracked, c->d_counters + 1, sizeof tracked, hipMemcpyDeviceToHost));
+        if ((r = stat_value(c, 0, tracked, cfg.stat_threshold, &out.stat))) return r;
+        if (out.stat <= cfg.stat_criterion || rounds >= max_rounds) break;
+        if ((uint64_t) (rounds + 1) * round_len + cfg.hist_offset >= (1ull << 32)) break;
+    }
+    unsigned long long cnt[3] = { 0, 0, 0 };
+    unsigned int       st     = 0;
+    HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost));
+    if (cfg.stat_criterion > 0 && rounds > 1) {
+        // calculate_average_results: scorers 0..n-3 scaled by N_target / N_tracked
+        // (mqi_tps_env.hpp:1430-1476, mqi_variables.hpp:51-66)
+        const double   w = (double) (p.total * 1.0 / (double) cnt[1]);
+        const uint64_t n = nvox(c) * (c->per_scen ? (uint64_t) c->n_scen : 1ull);
+        for (size_t k = 0; k + 2 < c->kinds.size(); ++k) KCHK(c, mqi_aux::scale(c->d_out[k], n, (double) (float) w, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    out.histories = cnt[1];
+    out.dropped   = cnt[2];
+    out.status    = st;
+    out.rounds    = rounds;
+    out.kernel_ms = ms;
+    if (res) *res = out;
+    if (st & MQI_STATUS_DIJ_FULL)
+        return fail(c, MQI_E_CAPACITY, "a Dij table overflowed: raise dij_capacity above %llu",
+                    (unsigned long long) c->dij_cap);
+    return 0;
+}
+
+int
+mqi_run_vertices(mqi_ctx* c, const float* vtx, uint64_t n, const uint32_t* spot_ids, const mqi_run_config* cfg_in,
+                 mqi_run_result* res) {
+    int r = ready(c);
+    if (r) return r;
+    if (!vtx || !cfg_in || n == 0) return fail(c, MQI_E_ARG, "bad vertex arguments");
+    if (n >= (1ull << 32)) return fail(c, MQI_E_ARG, "more than 2^32 vertices");
+    if (c->kinds.empty()) return fail(c, MQI_E_STATE, "no scorers");
+    hipSetDevice(c->device);
+    mqi_run_config cfg = *cfg_in;
+    if (cfg.dij) return fail(c, MQI_E_ARG, "Dij scoring needs a spot table (mqi_run)");
+    float*    d_v = nullptr;
+    uint32_t* d_s = nullptr;
+    if ((r = dalloc(c, &d_v, 7 * n))) return r;
+    HIPCHK(c, hipMemcpyAsync(d_v, vtx, sizeof(float) * 7 * n, hipMemcpyHostToDevice, c->stream));
+    if (spot_ids) {
+        if ((r = dalloc(c, &d_s, n))) return r;
+        HIPCHK(c, hipMemcpyAsync(d_s, spot_ids, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    const uint64_t one_end = n;
+    const uint32_t zero    = 0;
+    if ((r = ensure_items(c, 1))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->d_item_end, &one_end, sizeof one_end, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_item_spot, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_item_scen, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    transport_params P;
+    fill_params(c, P, cfg);
+    P.vertices    = d_v;
+    P.vertex_spot = d_s;
+    P.n_items     = 1;
+    P.total       = n;
+    KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 4, c->stream));
+    KCHK(c, mqi_aux::fill_u32(c->d_status, 0u, 1, c->stream));
+    KCHK(c, mqi_aux::fill_u64(c->d_hoff, cfg.hist_offset, 1, c->stream));
+    const int threads = cfg.block_size > 0 ? cfg.block_size : 256;
+    int       blocks;
+    if (cfg.batched) {
+        blocks = persistent_blocks(c, cfg, threads, 0);
+    } else {
+        uint64_t lanes = std::min<uint64_t>(n, 1ull << 18);
+        blocks         = (int) ((lanes + threads - 1) / threads);
+    }
+    if ((r = ensure_stack(c, (size_t) P.stack_cap * 9 * sizeof(float) * (size_t) blocks * threads))) return r;
+    P.stack = c->d_stack;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    KCHK(c, launch(c, P, cfg, cfg.batched ? 1 : 0, 0, blocks, threads));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    unsigned long long cnt[3] = { 0, 0, 0 };
+    HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
+    hipFree(d_v);
+    if (d_s) hipFree(d_s);
+    if (res) {
+        std::memset(res, 0, sizeof *res);
+        res->histories = cnt[1];
+        res->dropped   = cnt[2];
+        res->rounds    = 1;
+        res->kernel_ms = t;
+        res->stat      = -1.0;
+    }
+    return 0;
+}
+
+int
+mqi_trace(mqi_ctx* c, const mqi_run_config* cfg_in, const float* vertices, uint64_t n_hist, int max_steps,
+          int64_t* cnb, float* rec, int32_t* n_steps) {
+    int r = ready(c);
+    if (r) return r;
+    if (!cfg_in || !cnb || !rec || !n_steps || max_steps <= 0 || n_hist == 0)
+        return fail(c, MQI_E_ARG, "bad trace arguments");
+    hipSetDevice(c->device);
+    mqi_run_config cfg = *cfg_in;
+    run_plan       p;
+    float*         d_v = nullptr;
+    if (vertices) {
+        p.total = n_hist;
+        p.item_end.push_back(n_hist);
+        p.item_spot.push_back(0);
+        p.item_scen.push_back(0);
+        if ((r = dalloc(c, &d_v, 7 * n_hist))) return r;
+        HIPCHK(c, hipMemcpyAsync(d_v, vertices, sizeof(float) * 7 * n_hist, hipMemcpyHostToDevice, c->stream));
+    } else {
+        if (c->spots.empty()) return fail(c, MQI_E_STATE, "no spots");
+        p = make_plan(c, -1);
+        if (p.total != n_hist) return fail(c, MQI_E_ARG, "trace: plan has %llu histories, caller expects %llu",
+                                           (unsigned long long) p.total, (unsigned long long) n_hist);
+    }
+    if ((r = upload_plan(c, p))) return r;
+    int64_t* d_cnb = nullptr;
+    float*   d_rec = nullptr;
+    int32_t* d_n   = nullptr;
+    const uint64_t slots = n_hist * (uint64_t) max_steps;
+    if ((r = dalloc(c, &d_cnb, slots)) || (r = dalloc(c, &d_rec, slots * 12)) || (r = dalloc(c, &d_n, n_hist))) return r;
+    KCHK(c, mqi_aux::fill_u32(d_n, 0u, n_hist, c->stream));
+    transport_params P;
+    fill_params(c, P, cfg);
+    P.n_items   = (int) p.item_end.size();
+    P.total     = p.total;
+    P.trace_cnb = d_cnb;
+    P.trace_rec = d_rec;
+    P.trace_cap = max_steps;
+    P.trace_n   = d_n;
+    P.vertices  = d_v;
+    KCHK(c, mqi_aux::fill_u64(c->d_hoff, cfg.hist_offset, 1, c->stream));
+    KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 4, c->stream));
+    const int threads = 64;
+    const int blocks  = (int) ((n_hist + threads - 1) / threads);
+    if ((r = ensure_stack(c, (size_t) P.stack_cap * 9 * sizeof(float) * (size_t) blocks * threads))) return r;
+    P.stack = c->d_stack;
+    KCHK(c, launch(c, P, cfg, 0, 1, blocks, threads));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(cnb, d_cnb, sizeof(int64_t) * slots, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rec, d_rec, sizeof(float) * 12 * slots, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(n_steps, d_n, sizeof(int32_t) * n_hist, hipMemcpyDeviceToHost));
+    hipFree(d_cnb);
+    hipFree(d_rec);
+    hipFree(d_n);
+    if (d_v) hipFree(d_v);
+    return 0;
+}
+
+int
+mqi_get_scorer(mqi_ctx* c, int k, int scenario, double* out) {
+    int r = ready(c);
+    if (r) return r;
+    if (k < 0 || k >= (int) c->kinds.size() || !out) return fail(c, MQI_E_ARG, "no scorer %d", k);
+    if (scenario < 0 || scenario >= c->n_scen) return fail(c, MQI_E_ARG, "no scenario %d", scenario);
+    const uint64_t off = c->per_scen ? (uint64_t) scenario * nvox(c) : 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->d_out[k] + off, sizeof(double) * nvox(c), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int
+mqi_scorer_device_ptr(mqi_ctx* c, int k, int scenario, double** p) {
+    int r = ready(c);
+    if (r) return r;
+    if (k < 0 || k >= (int) c->kinds.size() || !p) return fail(c, MQI_E_ARG, "no scorer %d", k);
+    if (scenario < 0 || scenario >= c->n_scen) return fail(c, MQI_E_ARG, "no scenario %d", scenario);
+    *p = c->d_out[k] + (c->per_scen ? (uint64_t) scenario * nvox(c) : 0);
+    return 0;
+}
+
+int
+mqi_stat(mqi_ctx* c, int scenario, uint64_t n, float thr, double* out) {
+    int r = ready(c);
+    if (r) return r;
+    if (!out) return MQI_E_ARG;
+    return stat_value(c, scenario, n, thr, out);
+}
+
+int
+mqi_dij_build(mqi_ctx* c, double rel_thr) {
+    int r = ready(c);
+    if (r) return r;
+    if (!c->d_dij_keys) return fail(c, MQI_E_STATE, "no Dij tables: run with cfg.dij = 1 first");
+    hipSetDevice(c->device);
+    free_dij(c);
+    const uint32_t n_items = (uint32_t) c->dij_items;
+    const int      n_spots = (int) c->spots.size();
+    double*        d_max   = nullptr;
+    uint32_t*      d_cnt   = nullptr;
+    if ((r = dalloc(c, &d_max, n_items)) || (r = dalloc(c, &d_cnt, n_items))) return r;
+    KCHK(c, mqi_aux::dij_compact(c->d_dij_keys, c->d_dij_vals, c->dij_cap, n_items, rel_thr, d_max, d_cnt, c->stream));
+    std::vector<uint32_t> cnt(n_items);
+    HIPCHK(c, hipMemcpy(cnt.data(), d_cnt, sizeof(uint32_t) * n_items, hipMemcpyDeviceToHost));
+    c->dij.resize(c->n_scen);
+    for (int s = 0; s < c->n_scen; ++s) {
+        csr_dev& m = c->dij[s];
+        m.rows     = n_spots;
+        m.cols     = (int64_t) nvox(c);
+        std::vector<int64_t> indptr(n_spots + 1, 0);
+        for (int j = 0; j < n_spots; ++j) indptr[j + 1] = indptr[j] + cnt[(size_t) s * n_spots + j];
+        m.nnz = indptr[n_spots];
+        if ((r = dalloc(c, &m.indptr, n_spots + 1)) || (r = dalloc(c, &m.col, std::max<int64_t>(1, m.nnz))) ||
+            (r = dalloc(c, &m.val, std::max<int64_t>(1, m.nnz))))
+            return r;
+        HIPCHK(c, hipMemcpy(m.indptr, indptr.data(), sizeof(int64_t) * (n_spots + 1), hipMemcpyHostToDevice));
+        KCHK(c, mqi_aux::dij_write(c->d_dij_keys + (size_t) s * n_spots * c->dij_cap,
+                                   c->d_dij_vals + (size_t) s * n_spots * c->dij_cap, c->dij_cap, n_spots, rel_thr,
+                                   d_max + (size_t) s * n_spots, m.indptr, m.col, m.val, c->stream));
+        if (m.nnz > 0) {
+            // sort every row by voxel
+            int32_t* tk = nullptr;
+            float*   tv = nullptr;
+            if ((r = dalloc(c, &tk, m.nnz)) || (r = dalloc(c, &tv, m.nnz))) return r;
+            size_t bytes = 0;
+            KCHK(c, mqi_aux::segmented_sort(m.col, m.val, tk, tv, m.nnz, n_spots, m.indptr, m.indptr + 1, nullptr,
+                                            &bytes, c->stream));
+            char* scratch = nullptr;
+            if ((r = dalloc(c, &scratch, std::max<size_t>(bytes, 1)))) return r;
+            KCHK(c, mqi_aux::segmented_sort(m.col, m.val, tk, tv, m.nnz, n_spots, m.indptr, m.indptr + 1, scratch,
+                                            &bytes, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            hipFree(scratch);
+            hipFree(tk);
+            hipFree(tv);
+            // voxel-major copy for D w (counting sort by voxel)
+            uint32_t*           d_cc  = nullptr;
+            unsigned long long* d_cur = nullptr;
+            if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &d_cur, m.cols)) ||
+                (r = dalloc(c, &m.tindptr, m.cols + 1)) || (r = dalloc(c, &m.tcol, m.nnz)) ||
+                (r = dalloc(c, &m.tval, m.nnz)))
+                return r;
+            KCHK(c, mqi_aux::fill_u32(d_cc, 0u, m.cols, c->stream));
+            KCHK(c, mqi_aux::csr_transpose_count(m.col, m.nnz, d_cc, c->stream));
+            std::vector<uint32_t> cc(m.cols);
+            HIPCHK(c, hipMemcpy(cc.data(), d_cc, sizeof(uint32_t) * m.cols, hipMemcpyDeviceToHost));
+            std::vector<int64_t> tptr(m.cols + 1, 0);
+            for (int64_t v = 0; v < m.cols; ++v) tptr[v + 1] = tptr[v] + cc[v];
+            HIPCHK(c, hipMemcpy(m.tindptr, tptr.data(), sizeof(int64_t) * (m.cols + 1), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(d_cur, tptr.data(), sizeof(int64_t) * m.cols, hipMemcpyHostToDevice));
+            KCHK(c, mqi_aux::csr_transpose_scatter(m.indptr, m.col, m.val, m.rows, d_cur, m.tcol, m.tval, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            hipFree(d_cc);
+            hipFree(d_cur);
+        }
+    }
+    hipFree(d_max);
+    hipFree(d_cnt);
+    return 0;
+}
+
+int
+mqi_dij_info(mqi_ctx* c, int s, int64_t* nnz, int64_t* rows, int64_t* cols) {
+    if (!c) return MQI_E_ARG;
+    if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);
+    if (nnz) *nnz = c->dij[s].nnz;
+    if (rows) *rows = c->dij[s].rows;
+    if (cols) *cols = c->dij[s].cols;
+    return 0;
+}
+
+int
+mqi_dij_copy(mqi_ctx* c, int s, int64_t* indptr, int32_t* indices, float* values) {
+    if (!c) return MQI_E_ARG;
+    if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);
+    const csr_dev& m = c->dij[s];
+    if (indptr) HIPCHK(c, hipMemcpy(indptr, m.indptr, sizeof(int64_t) * (m.rows + 1), hipMemcpyDeviceToHost));
+    if (indices && m.nnz) HIPCHK(c, hipMemcpy(indices, m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
+    if (values && m.nnz) HIPCHK(c, hipMemcpy(values, m.val, sizeof(float) * m.nnz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int
+mqi_dij_device_ptrs(mqi_ctx* c, int s, const int64_t** indptr, const int32_t** indices, const float** values) {
+    if (!c) return MQI_E_ARG;
+    if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);
+    if (indptr) *indptr = c->dij[s].indptr;
+    if (indices) *indices = c->dij[s].col;
+    if (values) *values = c->dij[s].val;
+    return 0;
+}
+
+int
+mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_double, double alpha, int accumulate,
+               void* stream) {
+    if (!c || !x || !y) return MQI_E_ARG;
+    if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);
+    hipSetDevice(c->device);
+    const csr_dev& m  = c->dij[s];
+    hipStream_t    st = stream ? (hipStream_t) stream : c->stream;
+    HIPCHK(c, hipEventRecord(c->ev0, st));
+    if (transpose) {
+        // D^T g: rows of the spot-major matrix
+        KCHK(c, mqi_aux::csr_rows(m.indptr, m.col, m.val, x, y, y_double, m.rows, alpha, accumulate, 0, st));
+    } else {
+        // D w: rows of the voxel-major copy
+        if (m.nnz == 0) {
+            if (!accumulate) {
+                if (y_double)
+                    KCHK(c, mqi_aux::fill_u64(y, 0ull, m.cols, st));
+                else
+                    KCHK(c, mqi_aux::fill_u32(y, 0u, m.cols, st));
+            }
+        } else {
+            KCHK(c, mqi_aux::csr_rows(m.tindptr, m.tcol, m.tval, x, y, y_double, m.cols, alpha, accumulate, 0, st));
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0;
+    hipEventElapsedTime(&t, c->ev0, c->ev1);
+    c->last_matvec_ms = t;
+    return 0;
+}
+
+int
+mqi_last_matvec_ms(mqi_ctx* c, double* ms) {
+    if (!c || !ms) return MQI_E_ARG;
+    *ms = c->last_matvec_ms;
+    return 0;
+}
+
+int
+mqi_math_probe(mqi_ctx* c, int fn, const double* x, const double* y, int64_t n, double* out) {
+    if (!c || !x || !out || n <= 0) return MQI_E_ARG;
+    hipSetDevice(c->device);
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    int     r;
+    if ((r = dalloc(c, &dx, n)) || (r = dalloc(c, &dy, n)) || (r = dalloc(c, &dout, n))) return r;
+    HIPCHK(c, hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dy, y ? y : x, sizeof(double) * n, hipMemcpyHostToDevice));
+    KCHK(c, mqi_aux::math_probe(fn, dx, dy, dout, n, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost));
+    hipFree(dx);
+    hipFree(dy);
+    hipFree(dout);
+    return 0;
+}
+
+}   // extern "C"

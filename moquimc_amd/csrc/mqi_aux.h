@@ -1,0 +1,31 @@
+// Host-side launchers of the auxiliary kernels (mqi_aux.hip).
+#ifndef MQI_AUX_H
+#define MQI_AUX_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mqi_aux
+{
+int fill_u64(void* p, unsigned long long v, uint64_t n, hipStream_t s);
+int fill_u32(void* p, uint32_t v, uint64_t n, hipStream_t s);
+int hu_to_density(const int16_t* hu, float* rho, uint64_t n, float scale, hipStream_t s);
+int math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t s);
+int stat_reduce(const double* s1, const double* s2, uint64_t n, double N, float thr, float* d_max, double* d_sum,
+                unsigned long long* d_cnt, hipStream_t s);
+int scale(double* p, uint64_t n, double w, hipStream_t s);
+int dij_compact(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+                double* item_max, uint32_t* counts, hipStream_t s);
+int dij_write(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+              const double* item_max, const int64_t* row_start, int32_t* cols, float* out_vals, hipStream_t s);
+int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, int64_t nnz, int n_seg,
+                   const int64_t* seg_begin, const int64_t* seg_end, void* scratch, size_t* scratch_bytes,
+                   hipStream_t s);
+int csr_rows(const int64_t* indptr, const int32_t* col, const float* val, const float* x, void* y, int y_is_double,
+             int64_t n_rows, double alpha, int accumulate, int blocks, hipStream_t s);
+int csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t s);
+int csr_transpose_scatter(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
+                          unsigned long long* cursor, int32_t* tcol, float* tval, hipStream_t s);
+}   // namespace mqi_aux
+
+#endif

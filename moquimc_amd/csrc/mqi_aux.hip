@@ -1,0 +1,339 @@
+// Auxiliary kernels: buffer fills, HU -> density calibration, the
+// statistical-uncertainty reduction, Dij compaction into CSR, and the Dij
+// forward (D w) / adjoint (D^T g) sparse products.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "mqi_aux.h"
+#include "mqi_device.hpp"
+
+#define MQI_TABLE_QUALIFIER static __constant__
+#include "mqi_tables.inc"
+
+namespace mqi_aux
+{
+
+// ------------------------------------------------------------------ fills
+// Grid-stride fills; used instead of hipMemsetAsync so a captured graph
+// re-clears its buffers on every replay.
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n) {
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static int
+grid_for(uint64_t n) {
+    uint64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (int) b;
+}
+
+int
+fill_u64(void* p, unsigned long long v, uint64_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n)), dim3(256), 0, s, (unsigned long long*) p, v, n);
+    return (int) hipGetLastError();
+}
+int
+fill_u32(void* p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(256), 0, s, (uint32_t*) p, v, n);
+    return (int) hipGetLastError();
+}
+
+// ------------------------------------------------------------------ calibration
+// rho = hu_to_density(HU) * scale (mqi_tps_env.hpp:766-769)
+__global__ void k_hu_to_density(const int16_t* hu, float* rho, uint64_t n, float scale) {
+    __shared__ float corr[3996];
+    for (int i = threadIdx.x; i < 3996; i += blockDim.x) corr[i] = kDensityCorrection[i];
+    __syncthreads();
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
+        rho[i] = mqi_dev::hu_to_density(hu[i], corr) * scale;
+}
+
+int
+hu_to_density(const int16_t* hu, float* rho, uint64_t n, float scale, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_hu_to_density, dim3(grid_for(n)), dim3(256), 0, s, hu, rho, n, scale);
+    return (int) hipGetLastError();
+}
+
+// ------------------------------------------------------------------ math probe
+// The portable functions evaluated on the device (verification of the exact build).
+__global__ void k_math_probe(int fn, const double* x, const double* y, double* out, int64_t n) {
+    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r;
+    switch (fn) {
+    case 0: r = mqi_pm::pm_log(x[i]); break;
+    case 1: r = mqi_pm::pm_exp(x[i]); break;
+    case 2: r = mqi_pm::pm_pow(x[i], y[i]); break;
+    case 3: r = mqi_pm::pm_sin(x[i]); break;
+    case 4: r = mqi_pm::pm_cos(x[i]); break;
+    case 5: r = mqi_pm::pm_acos(x[i]); break;
+    case 6: r = mqi_pm::pm_sqrt(x[i]); break;
+    case 7: r = (double) ((float) x[i] / (float) y[i]); break;   // IEEE float division
+    case 8: r = x[i] / y[i]; break;                              // IEEE double division
+    default: r = (double) __builtin_sqrtf((float) x[i]); break;
+    }
+    out[i] = r;
+}
+
+int
+math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_math_probe, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, fn, x, y, out, n);
+    return (int) hipGetLastError();
+}
+
+// ------------------------------------------------------------------ statistics
+// calculate_standard_deviation (mqi_variables.hpp:21-48) fused with the
+// reduction of calculate_stat (mqi_tps_env.hpp:1396-1425):
+//   mean_i = S1_i / N, sd_i = sqrtf((S2_i/N - mean_i^2) / (N - 1)),
+//   stat = mean over {i : mean_i > thr * max mean} of sd_i / mean_i.
+// Pass 1: max of the float means.  Pass 2: sum and count above threshold.
+__global__ void k_stat_max(const double* s1, uint64_t n, double N, float* out_max) {
+    float m = 0.0f;
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x) {
+        float mu = (float) (s1[i] / N);
+        m        = mu > m ? mu : m;
+    }
+    typedef hipcub::BlockReduce<float, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    float bm = BR(tmp).Reduce(m, hipcub::Max());
+    if (threadIdx.x == 0) atomicMax((int*) out_max, __float_as_int(bm));   // non-negative floats order as ints
+}
+
+__global__ void k_stat_sum(const double* s1, const double* s2, uint64_t n, double N, const float* max_mu, float thr,
+                           double* out_sum, unsigned long long* out_cnt) {
+    const float lim = *max_mu * thr;
+    double      acc = 0.0;
+    unsigned long long cnt = 0;
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x) {
+        if (s1[i] == 0.0 && s2[i] == 0.0) continue;   // untouched slot (key1 == empty_pair)
+        float mu = (float) (s1[i] / N);
+        float sd = sqrtf((float) (((s2[i] / N) - ((s1[i] / N) * (s1[i] / N))) / (N - 1)));
+        if ((double) mu > (double) lim) {
+            acc += (double) (sd / mu);
+            cnt += 1;
+        }
+    }
+    typedef hipcub::BlockReduce<double, 256>             BRd;
+    typedef hipcub::BlockReduce<unsigned long long, 256> BRu;
+    __shared__ typename BRd::TempStorage t1;
+    __shared__ typename BRu::TempStorage t2;
+    double                               bs = BRd(t1).Sum(acc);
+    __syncthreads();
+    unsigned long long bc = BRu(t2).Sum(cnt);
+    if (threadIdx.x == 0) {
+        atomicAdd(out_sum, bs);
+        atomicAdd(out_cnt, bc);
+    }
+}
+
+int
+stat_reduce(const double* s1, const double* s2, uint64_t n, double N, float thr, float* d_max, double* d_sum,
+            unsigned long long* d_cnt, hipStream_t s) {
+    fill_u32(d_max, 0u, 1, s);
+    fill_u64(d_sum, 0ull, 1, s);
+    fill_u64(d_cnt, 0ull, 1, s);
+    int g = grid_for(n);
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_stat_max, dim3(g), dim3(256), 0, s, s1, n, N, d_max);
+    hipLaunchKernelGGL(k_stat_sum, dim3(g), dim3(256), 0, s, s1, s2, n, N, d_max, thr, d_sum, d_cnt);
+    return (int) hipGetLastError();
+}
+
+// rescale of scorers by N_target / N_tracked (calculate_average, mqi_variables.hpp:51-66)
+__global__ void k_scale(double* p, uint64_t n, double w) {
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
+        p[i] *= w;
+}
+int
+scale(double* p, uint64_t n, double w, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, s, p, n, w);
+    return (int) hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Dij compaction
+// One workgroup per item (spot of a scenario) scans its hash table.
+// Pass A: per-item maximum.  Pass B: count entries >= rel_thr * max.
+// Pass C: write (voxel, value) in slot order into the CSR row; rows are then
+// sorted by voxel with a segmented radix sort.
+__global__ void k_dij_max(const uint32_t* keys, const double* vals, uint64_t cap, double* item_max) {
+    const uint32_t item = blockIdx.x;
+    const uint32_t* k   = keys + (uint64_t) item * cap;
+    const double*   v   = vals + (uint64_t) item * cap;
+    double          m   = 0.0;
+    for (uint64_t i = threadIdx.x; i < cap; i += blockDim.x)
+        if (k[i] != 0xffffffffu && v[i] > m) m = v[i];
+    typedef hipcub::BlockReduce<double, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    double                              bm = BR(tmp).Reduce(m, hipcub::Max());
+    if (threadIdx.x == 0) item_max[item] = bm;
+}
+
+__global__ void k_dij_count(const uint32_t* keys, const double* vals, uint64_t cap, const double* item_max,
+                            double rel_thr, uint32_t* counts) {
+    const uint32_t item = blockIdx.x;
+    const uint32_t* k   = keys + (uint64_t) item * cap;
+    const double*   v   = vals + (uint64_t) item * cap;
+    const double    lim = item_max[item] * rel_thr;
+    uint32_t        c   = 0;
+    for (uint64_t i = threadIdx.x; i < cap; i += blockDim.x)
+        if (k[i] != 0xffffffffu && v[i] > 0.0 && v[i] >= lim) ++c;
+    typedef hipcub::BlockReduce<uint32_t, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    uint32_t                            bc = BR(tmp).Sum(c);
+    if (threadIdx.x == 0) counts[item] = bc;
+}
+
+__global__ void k_dij_write(const uint32_t* keys, const double* vals, uint64_t cap, const double* item_max,
+                            double rel_thr, const int64_t* row_start, int32_t* cols, float* out_vals) {
+    const uint32_t item = blockIdx.x;
+    const uint32_t* k   = keys + (uint64_t) item * cap;
+    const double*   v   = vals + (uint64_t) item * cap;
+    const double    lim = item_max[item] * rel_thr;
+    int64_t         base = row_start[item];
+    typedef hipcub::BlockScan<uint32_t, 256> BS;
+    __shared__ typename BS::TempStorage tmp;
+    __shared__ uint32_t                 chunk_total;
+    for (uint64_t i0 = 0; i0 < cap; i0 += blockDim.x) {
+        const uint64_t i    = i0 + threadIdx.x;
+        const bool     keep = i < cap && k[i] != 0xffffffffu && v[i] > 0.0 && v[i] >= lim;
+        uint32_t       pos, tot;
+        BS(tmp).ExclusiveSum(keep ? 1u : 0u, pos, tot);
+        if (keep) {
+            cols[base + pos]     = (int32_t) k[i];
+            out_vals[base + pos] = (float) v[i];
+        }
+        if (threadIdx.x == 0) chunk_total = tot;
+        __syncthreads();
+        base += chunk_total;
+        __syncthreads();
+    }
+}
+
+int
+dij_compact(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+            double* item_max, uint32_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_dij_max, dim3(n_items), dim3(256), 0, s, keys, vals, cap, item_max);
+    hipLaunchKernelGGL(k_dij_count, dim3(n_items), dim3(256), 0, s, keys, vals, cap, item_max, rel_thr, counts);
+    return (int) hipGetLastError();
+}
+
+int
+dij_write(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+          const double* item_max, const int64_t* row_start, int32_t* cols, float* out_vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_dij_write, dim3(n_items), dim3(256), 0, s, keys, vals, cap, item_max, rel_thr, row_start,
+                       cols, out_vals);
+    return (int) hipGetLastError();
+}
+
+int
+segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, int64_t nnz, int n_seg,
+               const int64_t* seg_begin, const int64_t* seg_end, void* scratch, size_t* scratch_bytes,
+               hipStream_t s) {
+    hipcub::DoubleBuffer<int32_t> k(keys, keys_tmp);
+    hipcub::DoubleBuffer<float>   v(vals, vals_tmp);
+    hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(scratch, *scratch_bytes, k, v, (int) nnz, n_seg,
+                                                               seg_begin, seg_end, 0, 32, s);
+    if (e != hipSuccess) return (int) e;
+    if (scratch && k.Current() != keys) {
+        hipMemcpyAsync(keys, k.Current(), sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(vals, v.Current(), sizeof(float) * nnz, hipMemcpyDeviceToDevice, s);
+    }
+    return (int) hipGetLastError();
+}
+
+// ------------------------------------------------------------------ sparse products
+// Row-wise CSR product y[r] = sum_k val[k] * x[col[k]] with FP64 accumulation.
+// One wavefront per row, rows interleaved over the grid; values and column
+// indices are streamed once, x is gathered (it stays in the last-level cache).
+template<typename TY>
+__global__ void __launch_bounds__(256) k_csr_rows(const int64_t* __restrict__ indptr, const int32_t* __restrict__ col,
+                                                  const float* __restrict__ val, const float* __restrict__ x,
+                                                  TY* __restrict__ y, int64_t n_rows, double alpha, int accumulate) {
+    const int     lane  = threadIdx.x & 63;
+    const int64_t wave  = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
+    for (int64_t r = wave; r < n_rows; r += waves) {
+        const int64_t b = indptr[r], e = indptr[r + 1];
+        double        acc = 0.0;
+        int64_t       k   = b + lane;
+        // 4 independent loads in flight per lane
+        for (; k + 192 < e; k += 256) {
+            const float   v0 = val[k], v1 = val[k + 64], v2 = val[k + 128], v3 = val[k + 192];
+            const int32_t c0 = col[k], c1 = col[k + 64], c2 = col[k + 128], c3 = col[k + 192];
+            acc += (double) v0 * (double) x[c0];
+            acc += (double) v1 * (double) x[c1];
+            acc += (double) v2 * (double) x[c2];
+            acc += (double) v3 * (double) x[c3];
+        }
+        for (; k < e; k += 64) acc += (double) val[k] * (double) x[col[k]];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) {
+            const double out = alpha * acc;
+            if (accumulate)
+                y[r] = (TY) ((double) y[r] + out);
+            else
+                y[r] = (TY) out;
+        }
+    }
+}
+
+int
+csr_rows(const int64_t* indptr, const int32_t* col, const float* val, const float* x, void* y, int y_is_double,
+         int64_t n_rows, double alpha, int accumulate, int blocks, hipStream_t s) {
+    if (n_rows <= 0) return 0;
+    if (blocks <= 0) {
+        int64_t b = (n_rows + 3) / 4;
+        blocks    = (int) (b > 16384 ? 16384 : b);
+    }
+    if (y_is_double)
+        hipLaunchKernelGGL(k_csr_rows<double>, dim3(blocks), dim3(256), 0, s, indptr, col, val, x, (double*) y, n_rows,
+                           alpha, accumulate);
+    else
+        hipLaunchKernelGGL(k_csr_rows<float>, dim3(blocks), dim3(256), 0, s, indptr, col, val, x, (float*) y, n_rows,
+                           alpha, accumulate);
+    return (int) hipGetLastError();
+}
+
+// Transpose of a CSR matrix by counting sort: voxel-major copy for D w.
+__global__ void k_count_cols(const int32_t* col, int64_t nnz, uint32_t* cnt) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t) gridDim.x * blockDim.x)
+        atomicAdd(cnt + col[i], 1u);
+}
+__global__ void k_scatter_t(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
+                            unsigned long long* cursor, int32_t* tcol, float* tval) {
+    const int     lane  = threadIdx.x & 63;
+    const int64_t wave  = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
+    for (int64_t r = wave; r < n_rows; r += waves)
+        for (int64_t k = indptr[r] + lane; k < indptr[r + 1]; k += 64) {
+            unsigned long long p = atomicAdd(cursor + col[k], 1ull);
+            tcol[p]              = (int32_t) r;
+            tval[p]              = val[k];
+        }
+}
+
+int
+csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t s) {
+    if (nnz <= 0) return 0;
+    hipLaunchKernelGGL(k_count_cols, dim3(grid_for((uint64_t) nnz)), dim3(256), 0, s, col, nnz, cnt);
+    return (int) hipGetLastError();
+}
+int
+csr_transpose_scatter(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
+                      unsigned long long* cursor, int32_t* tcol, float* tval, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_t, dim3(4096), dim3(256), 0, s, indptr, col, val, n_rows, cursor, tcol, tval);
+    return (int) hipGetLastError();
+}
+
+}   // namespace mqi_aux

@@ -1,0 +1,93 @@
+// Launch parameters shared by the host runtime (mqi_api.cpp) and the
+// transport kernels (mqi_transport.hip).  Device pointers only.
+#ifndef MQI_KERNELS_H
+#define MQI_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mqi_hip.h"
+
+#define MQI_MAX_SCORERS 6
+
+// scorer kinds, in the order tps_env lays scorers out (mqi_tps_env.hpp:788-960)
+enum {
+    MQI_KIND_DOSE        = 0,   // dose_to_water
+    MQI_KIND_LETD_NUM    = 1,   // LETd_weight1
+    MQI_KIND_LETD_DEN    = 2,   // LETd_weight2
+    MQI_KIND_STAT_DOSE   = 3,   // Dose_stat (dose_to_water)
+    MQI_KIND_STAT_DOSE2  = 4,   // DoseSquare_stat (dose_to_water_square)
+    MQI_KIND_DIJ         = 5    // dose_to_water keyed by (voxel, spot)
+};
+
+// scoring loop semantics (MQI_LOOPS_* of mqi_hip.h)
+enum {
+    MQI_LOOPS_PLAIN = MQI_LOOPS_REFERENCE_PLAIN,   // scorers 0..n-3 twice (mqi_transport.hpp:204-225)
+    MQI_LOOPS_STAT  = MQI_LOOPS_REFERENCE_STAT     // (mqi_transport.hpp:343-364)
+};
+
+#define MQI_STATUS_DIJ_FULL 1u
+#define MQI_STATUS_TRACK_CAP 2u
+#define MQI_MAX_TRACK_STEPS 200000
+
+struct transport_params {
+    // geometry (one voxel child of the world)
+    int          nx, ny, nz;
+    const float* xe;
+    const float* ye;
+    const float* ze;
+    const float* density;   // g/mm^3, x fastest
+    float        fwd[9], inv[9], trans[3];
+    // robust scenarios: per-scenario edge shift (mm, 3 per scenario) and density scale; may be null
+    const float* scen_shift;
+    const float* scen_scale;
+    // work list: items are (scenario, spot) pairs with item_end = inclusive prefix sum of histories
+    const mqi_spot* spots;
+    const float*    vertices;      // optional: 7 floats per history (ke, pos, dir), replaces sampling
+    const uint32_t* vertex_spot;   // optional: spot id per vertex (the scorer_offset_vector)
+    int             n_spots;
+    const uint64_t* item_end;
+    const uint32_t* item_spot;
+    const uint32_t* item_scenario;
+    int             n_items;
+    uint64_t        total;
+    unsigned long long* next;   // dynamic fetch counter
+    const uint64_t* hist_offset;   // device scalar added to every history index (graph-replay safe)
+    uint64_t        seed;
+    // scorers
+    int      n_scorers;
+    int      kinds[MQI_MAX_SCORERS];
+    double*  out[MQI_MAX_SCORERS];
+    uint64_t out_scenario_stride;   // 0: scenarios share the arrays; else per-scenario offset (elements)
+    int      loops;
+    uint32_t* dij_keys;
+    double*   dij_vals;
+    uint64_t  dij_capacity;   // slots per item
+    // secondary stack
+    float* stack;
+    int    stack_cap;
+    // counters
+    unsigned long long* tracked;
+    unsigned long long* drops;
+    unsigned int*       status;
+    // per-step trace (verification)
+    int64_t* trace_cnb;
+    float*   trace_rec;
+    int      trace_cap;
+    int*     trace_n;
+};
+
+namespace mqi_fast
+{
+int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int blocks, int threads,
+                     size_t shmem, hipStream_t stream);
+int occupancy(int debug, int dynamic, int trace, int threads, size_t shmem);
+}
+namespace mqi_exact
+{
+int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int blocks, int threads,
+                     size_t shmem, hipStream_t stream);
+int occupancy(int debug, int dynamic, int trace, int threads, size_t shmem);
+}
+
+#endif

@@ -1,0 +1,816 @@
+// Transport kernels for gfx950: fippel stepping, ionization with straggling,
+// multiple Coulomb scattering, p-p elastic, p-O elastic/inelastic, the
+// dose / LETd / statistics / Dij scorers, and the two launch forms:
+//
+//   batched  one persistent launch over every (scenario, spot) history of a
+//            plan; lanes pull histories from a device counter with one
+//            atomic per wavefront refill (ballot + popcount), so a lane whose
+//            track ends takes the next secondary or history on the next
+//            iteration while its neighbours keep stepping.
+//   plain    one launch per spot, histories split statically over threads
+//            (the reference's mc::transport_particles_patient launch,
+//            mqi_transport.hpp:113-250, mqi_tps_env.hpp:1058-1129).
+//
+// Compiled twice: MQI_EXACT=1 (namespace mqi_exact, -ffp-contract=off,
+// portable math: bit-identical to the CPU restatement) and MQI_EXACT=0
+// (namespace mqi_fast, device math library, FMA contraction).
+#include "mqi_device.hpp"
+#include "mqi_kernels.h"
+
+#define MQI_TABLE_QUALIFIER static __constant__
+#include "mqi_tables.inc"
+
+#if MQI_EXACT
+#define MQI_NS mqi_exact
+#else
+#define MQI_NS mqi_fast
+#endif
+
+using namespace mqi_dev;
+
+namespace MQI_NS
+{
+
+// ------------------------------------------------------------------ physics
+// Per-lane context: LDS tables, grid view, RNG, the lane's secondary stack.
+struct ctx {
+    const float* T;   // LDS tables
+    grid_view    g;
+    rng_t        rng;
+    float*       stack;   // HBM, stack_cap * 9 floats for this lane
+    int          sp;
+    int          cap;
+    const float* fwd;
+    const float* trans;
+    unsigned     drops;
+};
+
+// p_ionization_tabulated::cross_section (mqi_p_ionization.hpp:254-268): Ei=0.1, Ef=299.6, step 0.5
+MQI_DI float cs_ion(const ctx& c, float Ek, float rho) {
+    float cs = 0;
+    if (Ek >= 0.1f && Ek <= 299.6f) {
+        uint16_t idx0 = uint16_t((Ek - 0.1f) / 0.5f);
+        uint16_t idx1 = idx0 + 1;
+        float    x0   = 0.1f + idx0 * 0.5f;
+        float    x1   = x0 + 0.5f;
+        cs            = intpl(Ek, x0, x1, c.T[T_CS_ION + idx0], c.T[T_CS_ION + idx1]);
+    }
+    cs *= rho;
+    return cs;
+}
+// pp / p-O tables: Ek_min 0.5, Ek_max 300, dEk 0.5 (mqi_pp_elastic.hpp:221-235 and twins)
+MQI_DI float cs_tab(const ctx& c, int off, float Ek, float rho) {
+    float cs = 0;
+    if (Ek >= 0.5f && Ek <= 300.0f) {
+        uint16_t idx0 = uint16_t((Ek - 0.5f) / 0.5f);
+        uint16_t idx1 = idx0 + 1;
+        float    x0   = 0.5f + idx0 * 0.5f;
+        float    x1   = x0 + 0.5;
+        cs            = intpl(Ek, x0, x1, c.T[off + idx0], c.T[off + idx1]);
+    }
+    cs *= rho;
+    return cs;
+}
+// dEdx (mqi_p_ionization.hpp:271-286)
+MQI_DI float dedx(const ctx& c, float Ek) {
+    float pw = 0;
+    if (Ek >= 0.1f && Ek <= 299.6f) {
+        uint16_t idx0 = uint16_t((Ek - 0.1f) / 0.5f);
+        uint16_t idx1 = idx0 + 1;
+        float    x0   = 0.1f + idx0 * 0.5f;
+        float    x1   = x0 + 0.5f;
+        pw            = intpl(Ek, x0, x1, c.T[T_DEDX + idx0], c.T[T_DEDX + idx1]);
+    } else if (Ek < 0.1f && Ek > 0) {
+        pw = c.T[T_DEDX];
+    }
+    return -1.0 * pw;
+}
+
+// energy_loss + energy_straggling (mqi_p_ionization.hpp:297-345); the
+// residual-range walk stops at index 0 where the reference's uint16_t
+// counter would wrap past zero.
+template<bool DBG>
+MQI_DI float energy_loss(ctx& c, const rel_q& rel, float rho, float step_length) {
+    const float* R   = c.T + T_RANGE;
+    float        L_w = step_length * spr<DBG>(rho, rel.Ek) * rho / units::water_density;
+    uint16_t     n   = uint16_t((rel.Ek - 0.1f) / 0.5f);
+    float        x0  = 0.1f + n * 0.5f;
+    float        x1  = x0 + 0.5f;
+    if (x0 > rel.Ek) n -= 1;
+    if (x1 < rel.Ek) n += 1;
+    float r = intpl(rel.Ek, x0, x1, R[n], R[n + 1]);
+    if (r < L_w) return rel.Ek;
+    r -= L_w;
+    for (;;) {
+        if (r >= R[n]) break;
+        if (n == 0) break;
+        --n;
+        if (n == 0) break;
+    }
+    x0            = 0.1f + n * 0.5f;
+    x1            = x0 + 0.5f;
+    float dE_mean = rel.Ek - intpl(r, R[n], R[n + 1], x0, x1);
+    float Te      = (rel.Te_max >= 0.08511) ? 0.08511 : rel.Te_max;
+    float O_sq    = units::two_pi_re2_mc2_h2o * rho / units::water_density * L_w;
+    O_sq *= Te / rel.beta_sq * (1.0 - 0.5 * rel.beta_sq);
+    float ret = c.rng.normal(dE_mean, MQ_SQRTF(O_sq));
+    if (ret < 0) ret *= -1.0;
+    return ret;
+}
+
+// along_step (mqi_p_ionization.hpp:348-420)
+template<bool DBG>
+MQI_DI void along_step(ctx& c, trk& t, float len, float rho) {
+    rel_q rel = relq(t.ke0);
+    float dE  = energy_loss<DBG>(c, rel, rho, len);
+    float r   = 1.0;
+    if (dE >= t.ke0) {
+        r         = t.ke0 / dE;
+        t.stopped = true;
+    }
+    float P     = momentum(rel);
+    float X0    = radiation_length<DBG>(rho);
+    float th_sq = ((13.9f / P) * (13.9f / P) / rel.beta_sq) * len / X0;
+    float th    = MQ_SQRTF(th_sq);
+    th          = c.rng.normal(0, MQ_SQRTF(2.0f) * th);
+    if (th < 0) th *= -1.0;
+    float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
+    rotate_direction(t.d1, th, phi);
+    t.dE += dE * r;
+    update_pos(t, r * len);
+    t.ke1 -= dE * r;
+}
+
+// secondary placement: pos -> fwd (pos - T) + T, dir -> fwd dir (e.g. mqi_pp_elastic.hpp:190-196)
+MQI_DI v3 place(const ctx& c, v3 p) {
+    MQI_NO_CONTRACT
+    v3 q = mk(p.x - c.trans[0], p.y - c.trans[1], p.z - c.trans[2]);
+    q    = mv(c.fwd, q);
+    return mk(q.x + c.trans[0], q.y + c.trans[1], q.z + c.trans[2]);
+}
+MQI_DI v3 turn(const ctx& c, v3 d) { return mv(c.fwd, d); }
+
+// track_stack_t::push_secondary: dropped when the stack is full (mqi_track_stack.hpp:36-41).
+// A pushed track keeps vtx0, vtx1.ke and dE; vtx1 position/direction are
+// overwritten when it is popped, local_dE is always zero, status is CREATED.
+MQI_DI void push(ctx& c, float ke0, v3 p0, v3 d0, float ke1, float dE) {
+    if (c.sp < c.cap) {
+        float* s = c.stack + 9 * c.sp;
+        s[0]     = ke0;
+        s[1]     = p0.x;
+        s[2]     = p0.y;
+        s[3]     = p0.z;
+        s[4]     = d0.x;
+        s[5]     = d0.y;
+        s[6]     = d0.z;
+        s[7]     = ke1;
+        s[8]     = dE;
+        ++c.sp;
+    } else {
+        ++c.drops;
+    }
+}
+
+// p_ionization_tabulated::post_step: delta-ray (mqi_p_ionization.hpp:424-477)
+template<bool DBG>
+MQI_DI void post_ion(ctx& c, trk& t) {
+    rel_q       rel   = relq(t.ke1);
+    const float T_cut = DBG ? 0.08511 * 1.0 : 0.0815 * 1.0;
+    float       Te, n;
+    // acceptance probability is above 1/2 for any finite energy; the bound
+    // only ends the loop for non-finite input (which the reference would spin on)
+    for (int it = 0; it < 4096; ++it) {
+        n  = c.rng.uniform();
+        Te = T_cut * rel.Te_max;
+        Te /= ((1.0 - n) * rel.Te_max + n * T_cut);
+        if (c.rng.uniform() < 1.0 - rel.beta_sq * Te / rel.Te_max + Te * Te / (2.0 * rel.Et_sq)) break;
+    }
+    if (DBG) {
+        push(c, 0.0f, place(c, t.p0), turn(c, t.d0), 0.0f, Te);
+    } else {
+        t.dE += Te;
+    }
+    t.ke1 -= Te;
+}
+
+// pp_elastic_tabulated::post_step (mqi_pp_elastic.hpp:119-219)
+MQI_DI void post_pp(ctx& c, trk& t) {
+    rel_q rel       = relq(t.ke1);
+    float min_value = 0.5f / rel.Ek;
+    float u         = c.rng.uniform() * (1.0 - 2.0 * min_value) + min_value;
+    float E1        = rel.Et;
+    float dE        = rel.Ek * u;
+    float E3        = (rel.Ek - dE) + units::Mp;
+    float E4        = dE + units::Mp;
+    float P1        = momentum(rel);
+    float P3        = MQ_SQRTF(E3 * E3 - units::Mp_sq);
+    float P4        = MQ_SQRTF(E4 * E4 - units::Mp_sq);
+    float cos_th3   = E1 * E3 - units::Mp_sq - units::Mp * (E1 - E3);
+    cos_th3 /= (P1 * P3);
+    float cos_th34 = E3 * E4 - E1 * units::Mp;
+    cos_th34 /= (P3 * P4);
+    if (cos_th3 > 1.0)
+        cos_th3 = 1.0;
+    else if (cos_th3 < -1.0)
+        cos_th3 = -1.0;
+    if (cos_th34 > 1.0)
+        cos_th34 = 1.0;
+    else if (cos_th34 < -1.0)
+        cos_th34 = -1.0;
+    float th3 = MQ_ACOSF(cos_th3);
+    float th4 = th3 - MQ_ACOSF(cos_th34);
+    float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
+    t.ke1 -= dE;
+    rotate_direction(t.d1, th3, phi);
+    v3 dd = t.d1;
+    rotate_direction(dd, th4, phi);
+    push(c, dE, place(c, t.p1), turn(c, dd), dE, 0.0f);
+}
+
+// po_elastic::post_step (mqi_po_elastic.hpp:96-217); exponential not truncated (CPU semantics)
+template<bool DBG>
+MQI_DI void post_poe(ctx& c, trk& t) {
+    rel_q rel = relq(t.ke1);
+    if (rel.Ek <= 5.5f) {
+        float dE = rel.Ek;
+        if (DBG)
+            push(c, dE, place(c, t.p1), turn(c, t.d1), 0.0f, dE);
+        else
+            t.ldE += dE;
+        t.ke1 -= dE;
+        t.stopped = true;
+    } else {
+        float Tp_avg = 0.65 * MQ_EXPD(-0.0013 * rel.Ek);
+        Tp_avg -= 0.71 * MQ_EXPD(-0.0177 * rel.Ek);
+        float Tp_max = (2.0 * units::Mo * rel.beta_sq * rel.gamma_sq);
+        Tp_max /= (1.0 + 2.0 * rel.gamma * units::MoMp + units::MoMp_sq);
+        (void) Tp_max;
+        float avg     = (float) (1.0 / Tp_avg);
+        float uu      = c.rng.uniform();
+        float dE      = -MQ_LOGF(uu) / avg;
+        float E1      = rel.Ek * (rel.Ek + 2.0 * units::Mp);
+        float E3      = (rel.Ek - dE) * (rel.Ek - dE + 2.0 * units::Mp);
+        float cos_th3 = (E1 + E3 - dE * (dE + 2.0 * units::Mo)) / 2.0 / MQ_SQRTF(E1 * E3);
+        if (cos_th3 > 1.0) cos_th3 = 1.0;
+        if (cos_th3 < -1.0) cos_th3 = -1.0;
+        float th3 = MQ_ACOSF(cos_th3);
+        float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
+        if (DBG)
+            push(c, dE, place(c, t.p0), turn(c, t.d0), 0.0f, dE);
+        else
+            t.ldE += dE;
+        t.ke1 -= dE;
+        rotate_direction(t.d1, th3, phi);
+    }
+}
+
+// po_inelastic_tabulated::post_step (mqi_po_inelastic.hpp:158-288)
+template<bool DBG>
+MQI_DI void post_poi(ctx& c, trk& t) {
+    const float Ek = t.ke1;
+    float       Eb = 5.0f;
+    float       Er = Ek;
+    const float E_mini = 2.0f, E_ratio = 0.65f;
+    float       Prob_2nd, Prob_long, power;
+    if (Ek <= 215 && Ek > 200) {
+        Prob_2nd  = 0.78f;
+        Prob_long = Prob_2nd + (1 - Prob_2nd) * 0.9;
+        power     = 0.4f;
+    } else if (Ek > 215) {
+        Prob_2nd  = 0.78f;
+        Prob_long = Prob_2nd + (1 - Prob_2nd) * 1.0;
+        power     = 0.4f;
+    } else if (Ek <= 200 && Ek > 150) {
+        Prob_2nd  = 0.72f;
+        Prob_long = Prob_2nd + (1 - Prob_2nd) * 0.83;
+        power     = 0.45f;
+    } else {
+        Prob_2nd  = 0.7f;
+        Prob_long = Prob_2nd + (1 - Prob_2nd) * 0.83;
+        power     = 0.52f;
+    }
+    while ((Er - Eb) > E_mini) {
+        Er -= Eb;
+        float u  = c.rng.uniform();
+        float dE = MQ_POWF(u, power) * (Er - E_mini) + E_mini;
+        if (dE >= Er) dE = Er;
+        Er -= dE;
+        t.ke1 -= dE + Eb;
+        float zeta = c.rng.uniform();
+        if (zeta < Prob_2nd) {
+            float cos_th = (2.0 * dE / Ek - 1.0) + 2.0 * (1 - dE / Ek) * c.rng.uniform();
+            if (cos_th < -1) cos_th = -1;
+            if (cos_th > 1) cos_th = 1;
+            float th  = MQ_ACOSF(cos_th);
+            float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
+            v3    dd  = t.d1;
+            rotate_direction(dd, th, phi);
+            push(c, dE, place(c, t.p1), turn(c, dd), dE, 0.0f);
+        } else if (zeta < Prob_long) {
+            // long-range energy leaves the track
+        } else if (DBG) {
+            push(c, dE, place(c, t.p0), turn(c, t.d0), 0.0f, dE);
+        }
+        Eb *= E_ratio;
+    }
+    t.dE += Er;
+    t.ke1 -= Er;
+    t.stopped = true;
+}
+
+// fippel_physics::stepping (mqi_fippel_physics.hpp:68-216).  The three
+// branches' along_step calls are merged into one call with the branch's
+// length; the random-number order is unchanged.
+template<bool DBG>
+MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b) {
+    if (rho < 1.0e-7) {
+        update_pos(t, d2b);
+        return;
+    } else if (rho > 99.9) {
+        t.stopped = true;
+        return;
+    }
+    if (t.ke0 <= 0.5f) {
+        if (t.ke0 < 0) t.ke0 = 0;
+        t.dE += t.ke0;
+        t.ke1 -= t.ke0;
+        // last_step (mqi_p_ionization.hpp:481-490)
+        rel_q rel = relq(t.ke0);
+        float L_w = 0;
+        if (t.dE > 0) L_w = -t.dE / dedx(c, rel.Ek);
+        float step_length = L_w * units::water_density / (spr<DBG>(rho, t.ke0) * rho);
+        update_pos(t, step_length);
+        t.stopped = true;
+        return;
+    }
+    rel_q rel = relq(t.ke0);
+    float cms = 1.0f;
+    cms       = cms * spr<DBG>(rho, rel.Ek) * rho / units::water_density;
+    float max_loss_energy = -1.0 * cms * dedx(c, rel.Ek);
+    float cs1[4]  = { cs_ion(c, rel.Ek, rho), cs_tab(c, T_CS_PP, rel.Ek, rho), cs_tab(c, T_CS_POE, rel.Ek, rho),
+                      cs_tab(c, T_CS_POI, rel.Ek, rho) };
+    float cs1_sum = cs1[0] + cs1[1] + cs1[2] + cs1[3];
+    rel_q rel_de  = relq(t.ke0 - max_loss_energy);
+    float cs2[4]  = { cs_ion(c, rel_de.Ek, rho), cs_tab(c, T_CS_PP, rel_de.Ek, rho),
+                      cs_tab(c, T_CS_POE, rel_de.Ek, rho), cs_tab(c, T_CS_POI, rel_de.Ek, rho) };
+    float cs2_sum = cs2[0] + cs2[1] + cs2[2] + cs2[3];
+    const bool first = cs1_sum >= cs2_sum;
+    float cs_sum  = first ? cs1_sum : cs2_sum;
+    float cs0 = first ? cs1[0] : cs2[0], csa = first ? cs1[1] : cs2[1], csb = first ? cs1[2] : cs2[2],
+          csc = first ? cs1[3] : cs2[3];
+    float prob       = c.rng.uniform();
+    float mfp        = -1.0f * MQ_LOGF(prob) / cs_sum;
+    float step_limit = cms * units::water_density / (spr<DBG>(rho, rel.Ek) * rho);
+    int   br;
+    float len;
+    if (d2b < mfp && d2b < step_limit) {
+        br  = 0;
+        len = d2b;
+    } else if ((mfp < d2b || fabsf(mfp - d2b) < 1e-3f) &&
+               (mfp < step_limit || fabsf(mfp - step_limit) < 1e-3f)) {
+        br  = 1;
+        len = mfp;
+    } else {
+        br  = 2;
+        len = step_limit;
+    }
+    along_step<DBG>(c, t, len, rho);
+    if (br != 1) return;
+    if (t.ke1 <= 0.5f) return;
+    float u = cs_sum * c.rng.uniform();
+    t.d1    = t.d0;
+    if (u < cs0) {
+        post_ion<DBG>(c, t);
+    } else if (u < (cs0 + csa)) {
+        post_pp(c, t);
+    } else if (u < (cs0 + csa + csb)) {
+        post_poe<DBG>(c, t);
+    } else if (u < (cs0 + csa + csb + csc)) {
+        post_poi<DBG>(c, t);
+    }
+}
+
+// ------------------------------------------------------------------ scoring
+// hit functions (mqi_scorer_energy_deposit.hpp:43-139)
+template<bool DBG>
+MQI_DI double dose_hit(const trk& t, float density, float volume) {
+    if (density < 1.0e-7) return 0.0;
+    return (t.dE + t.ldE) * 1.60218e-10 / (volume * density * spr<DBG>(density, t.ke0));
+}
+MQI_DI double letd_hit(const trk& t, float density, bool numer) {
+    density *= 1000.0;
+    double length = (t.p1.x - t.p0.x) * (t.p1.x - t.p0.x);
+    length += (t.p1.y - t.p0.y) * (t.p1.y - t.p0.y);
+    length += (t.p1.z - t.p0.z) * (t.p1.z - t.p0.z);
+    length = MQ_SQRTD(length);
+    if (length <= 0) return 0.0;
+    double let = t.dE / length / density;
+    if (let >= 25.0) return 0;
+    return numer ? t.dE * let : t.dE * 1.0;
+}
+
+// Dij: per-spot open-addressing table of (voxel, dose) in HBM.  Keys are
+// claimed with one 32-bit CAS, values added with FP64 atomics.
+MQI_DI void dij_insert(const transport_params& P, uint64_t item, uint32_t voxel, double value) {
+    if (value <= 0) return;
+    const uint64_t cap  = P.dij_capacity;
+    uint32_t*      keys = P.dij_keys + (uint64_t) item * cap;
+    double*        vals = P.dij_vals + (uint64_t) item * cap;
+    uint32_t       h    = voxel * 2654435761u;
+    uint64_t       slot = ((uint64_t) h * cap) >> 32;
+    for (uint64_t probe = 0; probe < cap; ++probe) {
+        uint32_t prev = atomicCAS(keys + slot, 0xffffffffu, voxel);
+        if (prev == 0xffffffffu || prev == voxel) {
+            atomicAdd(vals + slot, value);
+            return;
+        }
+        slot = (slot + 1 == cap) ? 0 : slot + 1;
+    }
+    atomicOr(P.status, MQI_STATUS_DIJ_FULL);
+}
+
+template<bool DBG>
+MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float density, float volume,
+                  uint64_t dij_item, uint64_t scen_off) {
+    const int     n   = P.n_scorers;
+    const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT
+    double        v[MQI_MAX_SCORERS];
+    for (int s = 0; s < n; ++s) {
+        const int k = P.kinds[s];
+        if (k == MQI_KIND_LETD_NUM)
+            v[s] = letd_hit(t, density, true);
+        else if (k == MQI_KIND_LETD_DEN)
+            v[s] = letd_hit(t, density, false);
+        else {
+            double d = dose_hit<DBG>(t, density, volume);
+            v[s]     = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
+        }
+    }
+    for (int s = 0; s < n; ++s) {
+        const int  k    = P.kinds[s];
+        const bool stat = (k == MQI_KIND_STAT_DOSE || k == MQI_KIND_STAT_DOSE2);
+        int        reps = 0;
+        if (P.loops == MQI_LOOPS_ONCE) {
+            reps = (stat ? roi >= 0 : roi > 0) ? 1 : 0;
+        } else if (P.loops == MQI_LOOPS_PLAIN) {
+            // mqi_transport.hpp:204-225: scorers 0..n-3 in both loops
+            reps = (roi > 0) ? ((s < n - 2) ? 2 : 1) : 0;
+        } else {
+            // mqi_transport.hpp:343-364: last two through get_mask_idx >= 0
+            if (s < n - 2)
+                reps = roi > 0 ? 1 : 0;
+            else if (n >= 2)
+                reps = roi >= 0 ? 1 : 0;
+        }
+        if (v[s] <= 0) continue;
+        if (k == MQI_KIND_DIJ) {
+            for (int r = 0; r < reps; ++r) dij_insert(P, dij_item, (uint32_t) cnb, v[s]);
+            continue;
+        }
+        for (int r = 0; r < reps; ++r) atomicAdd(P.out[s] + scen_off + cnb, v[s]);
+    }
+}
+
+// ------------------------------------------------------------------ lane machine
+struct lane {
+    trk      t;
+    bool     active;
+    uint32_t item;       // work item (spot of a scenario) of the current history
+    uint64_t hist;       // history index within the item
+    uint64_t g;          // history index in the flattened list
+    int      nsteps;     // trace mode: steps logged for this history
+    float    scale;      // scenario density scale
+    uint64_t scen_off;   // scenario offset into the output arrays
+    uint64_t dij_item;   // Dij table of (scenario, spot): scenario * n_spots + spot
+    int      tsteps;     // steps of the current track
+};
+
+// pop + enter the child grid: transport loop head (mqi_transport.hpp:158-191)
+MQI_DI bool enter(const transport_params& P, ctx& c, trk& t) {
+    MQI_NO_CONTRACT
+    v3 pp = mk(t.p0.x - P.trans[0], t.p0.y - P.trans[1], t.p0.z - P.trans[2]);
+    t.p0  = mv(P.inv, pp);
+    t.d0  = mv(P.inv, t.d0);
+    normalize(t.d0);
+    t.p1 = t.p0;
+    t.d1 = t.d0;
+    int cx, cy, cz;
+    index3(c.g, t.p0, t.d0, cx, cy, cz);
+    if (!is_valid(c.g, cx, cy, cz)) {
+        int   ox, oy, oz;
+        float dist = intersect_out(c.g, t.p0, t.d0, ox, oy, oz);
+        t.its      = dist;
+        if (dist < 0) return false;
+        update_pos(t, dist);
+        move(t);
+        index3(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
+    } else {
+        t.its = 0.0f;
+        t.cx  = cx;
+        t.cy  = cy;
+        t.cz  = cz;
+    }
+    return is_valid(c.g, t.cx, t.cy, t.cz) && !t.stopped;
+}
+
+MQI_DI void pop(ctx& c, trk& t) {
+    --c.sp;
+    const float* s = c.stack + 9 * c.sp;
+    t.ke0          = s[0];
+    t.p0           = mk(s[1], s[2], s[3]);
+    t.d0           = mk(s[4], s[5], s[6]);
+    t.ke1          = s[7];
+    t.dE           = s[8];
+    t.ldE          = 0.0f;
+    t.stopped      = false;
+}
+
+// Vertex of history h of item (scenario, spot): phsp_6d + norm_1d/const_1d
+// beamlet, then rotation/translation (mqi_beamlet.hpp:81-90, mqi_phsp6d.hpp:62-79).
+MQI_DI void sample_vertex(const transport_params& P, const mqi_spot& sp, uint32_t spot_id, uint32_t scenario,
+                          uint64_t h, trk& t) {
+    rng_t r;
+    r.s.init(P.seed, (uint32_t) h, spot_id, scenario, mqi_rng::STREAM_VERTEX);
+    float Ux = r.gauss();
+    float Vx = r.gauss();
+    float Uy = r.gauss();
+    float Vy = r.gauss();
+    float Uz = r.gauss();
+    float ke = sp.energy;
+    if (!sp.energy_const) {
+        float z = r.gauss();
+        ke      = z * sp.sigma_energy + sp.energy;
+    }
+    float ph[6];
+    for (int k = 0; k < 6; ++k) ph[k] = sp.mean[k];
+    ph[0] += sp.sigma[0] * Ux;
+    ph[1] += sp.sigma[1] * Uy;
+    ph[2] += sp.sigma[2] * Uz;
+    const float rho0 = sp.corr[0], rho1 = sp.corr[1];
+    ph[3] += sp.sigma[3] * (rho0 * Ux + Vx * MQ_SQRTD(1.0 - rho0 * rho0));
+    ph[4] += sp.sigma[4] * (rho1 * Uy + Vy * MQ_SQRTD(1.0 - rho1 * rho1));
+    ph[5]  = -1.0 * MQ_SQRTD(1.0 - ph[3] * ph[3] - ph[4] * ph[4]);
+    v3 pos = mv(sp.rotation, mk(ph[0], ph[1], ph[2]));
+    t.p0   = mk(pos.x + sp.translation[0], pos.y + sp.translation[1], pos.z + sp.translation[2]);
+    t.d0   = mv(sp.rotation, mk(ph[3], ph[4], ph[5]));
+    t.ke0  = ke;
+    t.ke1  = ke;
+    t.p1   = t.p0;
+    t.d1   = t.d0;
+    t.dE   = 0;
+    t.ldE  = 0;
+    t.stopped = false;
+}
+
+// history g of the flattened work list -> (item, history-in-item)
+MQI_DI void locate(const transport_params& P, uint64_t g, uint32_t& item, uint64_t& h) {
+    int lo = 0, hi = P.n_items - 1;   // first item with item_end > g
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (P.item_end[mid] > g)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    item = (uint32_t) lo;
+    h    = g - (lo ? P.item_end[lo - 1] : 0);
+}
+
+MQI_DI void trace_record(const transport_params& P, lane& L, uint64_t cnb) {
+    // per-history slots: history index g of the flattened list
+    const uint64_t g = L.g;
+    if (L.nsteps < P.trace_cap) {
+        const uint64_t k = g * P.trace_cap + L.nsteps;
+        P.trace_cnb[k]   = (int64_t) cnb;
+        float* r         = P.trace_rec + 12 * k;
+        r[0]             = L.t.its;
+        r[1]             = L.t.p0.x;
+        r[2]             = L.t.p0.y;
+        r[3]             = L.t.p0.z;
+        r[4]             = L.t.d0.x;
+        r[5]             = L.t.d0.y;
+        r[6]             = L.t.d0.z;
+        r[7]             = L.t.p1.x;
+        r[8]             = L.t.p1.y;
+        r[9]             = L.t.p1.z;
+        r[10]            = L.t.ke0;
+        r[11]            = L.t.dE;
+    }
+    L.nsteps++;
+    P.trace_n[g] = L.nsteps;
+}
+
+// one voxel-loop iteration (mqi_transport.hpp:192-233)
+template<bool DBG, bool TRACE>
+MQI_DI void step(const transport_params& P, ctx& c, lane& L) {
+    trk&           t   = L.t;
+    const uint64_t cnb = (uint64_t) (int32_t) (t.cz * c.g.nx * c.g.ny + t.cy * c.g.nx + t.cx);
+    t.its              = intersect_in(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
+    const float rho    = P.density[cnb] * L.scale;
+    stepping<DBG>(c, t, rho, t.its);
+    if (t.its < 0) {
+        L.active = false;
+        return;
+    }
+    if (TRACE) {
+        trace_record(P, L, cnb);
+    } else {
+        const uint64_t nxy = (uint64_t) (c.g.nx * c.g.ny);
+        const int      k = (int) (cnb / nxy), j = (int) ((cnb % nxy) / c.g.nx), i = (int) ((cnb % nxy) % c.g.nx);
+        float          vol = c.g.X(i + 1) - c.g.X(i);
+        vol *= c.g.Y(j + 1) - c.g.Y(j);
+        vol *= c.g.Z(k + 1) - c.g.Z(k);
+        score<DBG>(P, t, cnb, rho, vol, L.dij_item, L.scen_off);
+    }
+    if (!t.stopped) {
+        step_axis(t.p1.x, t.d1.x, c.g.xe, c.g.sx, t.cx);
+        step_axis(t.p1.y, t.d1.y, c.g.ye, c.g.sy, t.cy);
+        step_axis(t.p1.z, t.d1.z, c.g.ze, c.g.sz, t.cz);
+        move(t);
+    }
+    L.active = is_valid(c.g, t.cx, t.cy, t.cz) && !t.stopped;
+    // a finite track takes a few thousand steps at most; the cap only ends
+    // tracks whose state went non-finite
+    if (++L.tsteps > MQI_MAX_TRACK_STEPS && L.active) {
+        L.active = false;
+        atomicOr(P.status, MQI_STATUS_TRACK_CAP);
+    }
+}
+
+// LDS staging of the tables and edges
+MQI_DI void stage(const transport_params& P, float* sh) {
+    for (int i = threadIdx.x; i < 600; i += blockDim.x) {
+        sh[T_CS_ION + i] = kCsPIon[i];
+        sh[T_DEDX + i]   = kDedxRestricted[i];
+        sh[T_RANGE + i]  = kRangeSteps[i];
+        sh[T_CS_PP + i]  = kCsPPElastic[i];
+        sh[T_CS_POE + i] = kCsPOElastic[i];
+        sh[T_CS_POI + i] = kCsPOInelastic[i];
+    }
+    float* xe = sh + T_TABLES;
+    float* ye = xe + P.nx + 1;
+    float* ze = ye + P.ny + 1;
+    for (int i = threadIdx.x; i <= P.nx; i += blockDim.x) xe[i] = P.xe[i];
+    for (int i = threadIdx.x; i <= P.ny; i += blockDim.x) ye[i] = P.ye[i];
+    for (int i = threadIdx.x; i <= P.nz; i += blockDim.x) ze[i] = P.ze[i];
+    __syncthreads();
+}
+
+// The transport kernel.  DYNAMIC: histories from the device counter
+// (batched persistent form); otherwise each thread takes a contiguous slice
+// [start, start+len) of the work list (plain form; start_and_length,
+// mqi_utils.hpp:138-147).
+template<bool DBG, bool DYNAMIC, bool TRACE>
+__global__ void __launch_bounds__(256) k_transport(transport_params P) {
+    extern __shared__ float sh[];
+    stage(P, sh);
+    ctx c;
+    c.T     = sh;
+    c.g.nx  = P.nx;
+    c.g.ny  = P.ny;
+    c.g.nz  = P.nz;
+    c.g.xe  = sh + T_TABLES;
+    c.g.ye  = c.g.xe + P.nx + 1;
+    c.g.ze  = c.g.ye + P.ny + 1;
+    c.fwd   = P.fwd;
+    c.trans = P.trans;
+    c.cap   = P.stack_cap;
+    c.sp    = 0;
+    c.drops = 0;
+    const uint64_t tid      = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nthreads = (uint64_t) gridDim.x * blockDim.x;
+    c.stack                 = P.stack + tid * (uint64_t) P.stack_cap * 9;
+
+    uint64_t s_next = 0, s_end = 0;
+    if (!DYNAMIC) {   // start_and_length over the flattened list
+        const uint64_t q = P.total / nthreads, r = P.total % nthreads;
+        s_next           = q * tid + (tid < r ? tid : r);
+        s_end            = s_next + q + (tid < r ? 1 : 0);
+    }
+    lane     L;
+    L.active = false;
+    L.item   = 0;
+    L.hist   = 0;
+    L.nsteps   = 0;
+    L.scale    = 1.0f;
+    L.scen_off = 0;
+    c.g.sx = c.g.sy = c.g.sz = 0.0f;
+    unsigned long long done = 0;
+    uint32_t           scen = 0;
+    const uint64_t     hoff = P.hist_offset ? *P.hist_offset : 0;
+
+    for (;;) {
+        if (!L.active && c.sp == 0) {
+            uint64_t g;
+            if (DYNAMIC) {
+                const unsigned long long m    = __ballot(1);
+                const int                lead = __builtin_ctzll(m);
+                const int                me   = __lane_id();
+                const int                rank = __popcll(m & ((1ull << me) - 1ull));
+                unsigned long long       base = 0;
+                if (me == lead) base = atomicAdd(P.next, (unsigned long long) __popcll(m));
+                base = __shfl(base, lead);
+                g    = base + rank;
+                if (g >= P.total) break;
+            } else {
+                if (s_next >= s_end) break;
+                g = s_next++;
+            }
+            L.g = g;
+            uint32_t spot_id;
+            if (P.vertices) {
+                // caller-made vertex (upload_vertices path); scenario 0
+                spot_id  = P.vertex_spot ? P.vertex_spot[g] : 0;
+                L.item   = spot_id;
+                L.hist   = g;
+                scen     = 0;
+            } else {
+                locate(P, g, L.item, L.hist);
+                spot_id = P.item_spot[L.item];
+                scen    = P.item_scenario[L.item];
+            }
+            c.g.sx                 = P.scen_shift ? P.scen_shift[3 * scen] : 0.0f;
+            c.g.sy                 = P.scen_shift ? P.scen_shift[3 * scen + 1] : 0.0f;
+            c.g.sz                 = P.scen_shift ? P.scen_shift[3 * scen + 2] : 0.0f;
+            L.scale                = P.scen_scale ? P.scen_scale[scen] : 1.0f;
+            L.scen_off             = (uint64_t) scen * P.out_scenario_stride;
+            L.dij_item             = (uint64_t) scen * P.n_spots + spot_id;
+            if (P.vertices) {
+                const float* v = P.vertices + 7 * g;
+                L.t.ke0        = v[0];
+                L.t.p0         = mk(v[1], v[2], v[3]);
+                L.t.d0         = mk(v[4], v[5], v[6]);
+                L.t.ke1        = L.t.ke0;
+                L.t.p1         = L.t.p0;
+                L.t.d1         = L.t.d0;
+                L.t.dE         = 0;
+                L.t.ldE        = 0;
+                L.t.stopped    = false;
+            } else {
+                sample_vertex(P, P.spots[spot_id], spot_id, scen, L.hist + hoff, L.t);
+            }
+            c.rng.s.init(P.seed, (uint32_t) (L.hist + hoff), spot_id, scen, mqi_rng::STREAM_TRANSPORT);
+            L.nsteps = 0;
+            L.tsteps = 0;
+            ++done;
+            L.active = enter(P, c, L.t);
+            if (!L.active) continue;
+        } else if (!L.active) {
+            pop(c, L.t);
+            L.tsteps = 0;
+            L.active = enter(P, c, L.t);
+            if (!L.active) continue;
+        }
+        step<DBG, TRACE>(P, c, L);
+    }
+    atomicAdd(P.tracked, done);
+    if (c.drops) atomicAdd(P.drops, (unsigned long long) c.drops);
+}
+
+int
+launch_transport(const transport_params& P, int debug, int dynamic, int trace, int blocks, int threads,
+                 size_t shmem, hipStream_t stream) {
+#define MQI_LAUNCH(D, Y, T)                                                                         \
+    hipLaunchKernelGGL((k_transport<D, Y, T>), dim3(blocks), dim3(threads), shmem, stream, P)
+    if (debug) {
+        if (dynamic) {
+            if (trace) MQI_LAUNCH(true, true, true); else MQI_LAUNCH(true, true, false);
+        } else {
+            if (trace) MQI_LAUNCH(true, false, true); else MQI_LAUNCH(true, false, false);
+        }
+    } else {
+        if (dynamic) {
+            if (trace) MQI_LAUNCH(false, true, true); else MQI_LAUNCH(false, true, false);
+        } else {
+            if (trace) MQI_LAUNCH(false, false, true); else MQI_LAUNCH(false, false, false);
+        }
+    }
+#undef MQI_LAUNCH
+    return (int) hipGetLastError();
+}
+
+// resident workgroups per CU of one instantiation
+int
+occupancy(int debug, int dynamic, int trace, int threads, size_t shmem) {
+    int         n = 0;
+    const void* f = nullptr;
+#define MQI_F(D, Y, T) f = reinterpret_cast<const void*>(&k_transport<D, Y, T>)
+    if (debug) {
+        if (dynamic) {
+            if (trace) MQI_F(true, true, true); else MQI_F(true, true, false);
+        } else {
+            if (trace) MQI_F(true, false, true); else MQI_F(true, false, false);
+        }
+    } else {
+        if (dynamic) {
+            if (trace) MQI_F(false, true, true); else MQI_F(false, true, false);
+        } else {
+            if (trace) MQI_F(false, false, true); else MQI_F(false, false, false);
+        }
+    }
+#undef MQI_F
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, shmem) != hipSuccess) return 1;
+    return n < 1 ? 1 : n;
+}
+
+}   // namespace MQI_NS

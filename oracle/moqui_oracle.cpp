@@ -1060,6 +1060,7 @@ struct Transport {
     Scorers*                  sc;
     int                       kernel;
     StepLog*                  log;
+    int64_t                   cur_hist = 0;
 
     Transport(const Grid<R>* grid, Rng* r, Scorers* s, int k, StepLog* l) :
         g(grid), stk(DBG ? 200 : 10), sc(s), kernel(k), log(l) {
@@ -1087,7 +1088,7 @@ struct Transport {
         if (log) {
             if (log->n < log->cap) {
                 log->cnb[log->n] = (int64_t) cnb;
-                float* r         = log->rec + 12 * log->n;
+                float* r         = log->rec + 13 * log->n;
                 r[0]             = (float) t.its_dist;
                 r[1]             = (float) t.v0.pos.x;
                 r[2]             = (float) t.v0.pos.y;
@@ -1100,12 +1101,23 @@ struct Transport {
                 r[9]             = (float) t.v1.pos.z;
                 r[10]            = (float) t.v0.ke;
                 r[11]            = (float) t.dE;
+                r[12]            = (float) cur_hist;
             }
             log->n++;
             return;
         }
         const int n  = (int) sc->kinds.size();
         const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT: idx(v) = v
+        if (kernel == 2) {
+            // each scorer once: regular scorers through idx(cnb) > 0, the two
+            // stat scorers through get_mask_idx(cnb) >= 0 (what the _stat
+            // kernel does; for fewer than three scorers also the plain kernel)
+            for (int s = 0; s < n; ++s) {
+                const bool stat = sc->kinds[s] >= 3;
+                if (stat ? roi >= 0 : roi > 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
+            }
+            return;
+        }
         // first loop: scorers 0 .. n-3 (mqi_transport.hpp:204-214 / 343-352)
         for (int s = 0; s < n - 2; ++s)
             if (roi > 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
@@ -1326,6 +1338,7 @@ run_job(const Job& J) {
                 spot           = J.spot_ids ? J.spot_ids[h] : 0;
             }
             rng.seed_history(J.master_seed, hin, spot, J.scenario, seed);
+            tr.cur_hist = h;
             tr.run_history(v);
             counts[t]++;
         }

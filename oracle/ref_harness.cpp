@@ -77,16 +77,17 @@ typedef float R;
 struct step_log {
     int64_t  cap = 0, n = 0;
     int64_t* cnb  = nullptr;   // voxel of the step
-    float*   rec  = nullptr;   // 12 floats per step: its.dist, vtx0.pos(3), vtx0.dir(3), vtx1.pos(3), ke0, dE
+    float*   rec  = nullptr;   // 13 floats per step: its.dist, vtx0.pos(3), vtx0.dir(3), vtx1.pos(3), ke0, dE, history
 };
-thread_local step_log* g_log = nullptr;
+thread_local step_log* g_log  = nullptr;
+thread_local int64_t   g_hist = 0;
 
 double
 log_hit(const mqi::track_t<R>& trk, const mqi::cnb_t& cnb, mqi::grid3d<mqi::density_t, R>&, mqi::material_t<R>*&) {
     step_log* L = g_log;
     if (L && L->n < L->cap) {
         L->cnb[L->n] = (int64_t) cnb;
-        float* r     = L->rec + 12 * L->n;
+        float* r     = L->rec + 13 * L->n;
         r[0]         = trk.its.dist;
         r[1]         = trk.vtx0.pos.x;
         r[2]         = trk.vtx0.pos.y;
@@ -99,6 +100,7 @@ log_hit(const mqi::track_t<R>& trk, const mqi::cnb_t& cnb, mqi::grid3d<mqi::dens
         r[9]         = trk.vtx1.pos.z;
         r[10]        = trk.vtx0.ke;
         r[11]        = trk.dE;
+        r[12]        = (float) g_hist;
     }
     if (L) L->n++;
     return 0.0;
@@ -393,6 +395,7 @@ ref_transport(int nx, int ny, int nz, const float* xe, const float* ye, const fl
         uint32_t    tracked_local = 0;
         for (int64_t h = h0; h < h1; ++h) {
             th.rnd_generator.seed((uint32_t) seeds[h]);
+            g_hist = h;
             mqi::vertex_t<R> v;
             v.ke  = vtx[7 * h];
             v.pos = mqi::vec3<R>(vtx[7 * h + 1], vtx[7 * h + 2], vtx[7 * h + 3]);

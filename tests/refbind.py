@@ -116,13 +116,14 @@ def ref_transport(debug, grid, vtx, seeds, spot_ids=None, mask=SC_DOSE, kernel=0
 
 
 def ref_trace(debug, grid, vtx, cap=200000):
-    """Per-step log of the reference transport (vacuum or not): cnb and 12 floats/step."""
+    """Per-step log of the reference transport (vacuum or not): cnb and 13 floats/step
+    (its.dist, vtx0.pos, vtx0.dir, vtx1.pos, ke0, dE, history)."""
     lib = reference(debug)
     vtx = np.ascontiguousarray(vtx, np.float32)
     n = vtx.shape[0]
     seeds = np.arange(1, n + 1, dtype=np.int32)
     cnb = np.zeros(cap, np.int64)
-    rec = np.zeros((cap, 12), np.float32)
+    rec = np.zeros((cap, 13), np.float32)
     nlog = np.zeros(1, np.int64)
     lib.ref_transport(grid.nx, grid.ny, grid.nz, ptr(grid.xe, c_f), ptr(grid.ye, c_f), ptr(grid.ze, c_f),
                       ptr(grid.density, c_f), ptr(grid.rot9, c_f), ptr(grid.t3, c_f), ctypes.c_int64(n),
@@ -156,7 +157,7 @@ def oracle_trace(grid, vtx, cap=200000, debug=1, prec=32, rng=0, math=0):
     n = vtx.shape[0]
     seeds = np.arange(1, n + 1, dtype=np.int32)
     cnb = np.zeros(cap, np.int64)
-    rec = np.zeros((cap, 12), np.float32)
+    rec = np.zeros((cap, 13), np.float32)
     nlog = np.zeros(1, np.int64)
     lib.oracle_transport(prec, rng, math, debug, ctypes.c_uint64(0), grid.nx, grid.ny, grid.nz,
                          ptr(grid.xe, c_f), ptr(grid.ye, c_f), ptr(grid.ze, c_f), ptr(grid.density, c_f),

@@ -1,0 +1,165 @@
+"""GPU verification build against the CPU restatement and the reference.
+
+The verification kernels (exact=1) use the same Philox stream and the same
+portable math as the oracle with rng=1, math=1, so every history follows the
+same trajectory; doses agree to FP64 summation order.
+"""
+import numpy as np
+import pytest
+
+import phantoms
+import refbind as rb
+
+pytestmark = pytest.mark.gpu
+
+
+def test_math_probe_bitwise(ctx):
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(1e-6, 10, 4000), rng.uniform(1e-30, 1e-3, 500), [1.0, 0.5, 2.0, 1e-300, 5e-324]])
+    xs = np.concatenate([rng.uniform(-8, 8, 4000), rng.uniform(-700, 700, 500)])
+    ang = np.concatenate([rng.uniform(0, 2 * np.pi, 4000), rng.uniform(-50, 50, 500)])
+    cosv = rng.uniform(-1, 1, 4000)
+    y = rng.uniform(-2, 2, len(x))
+    lib = rb.oracle()
+    cases = [(0, x, None), (1, xs, None), (2, x, y), (3, ang, None), (4, ang, None), (5, cosv, None), (6, x, None)]
+    for fn, a, b in cases:
+        a = np.ascontiguousarray(a, np.float64)
+        bb = a if b is None else np.ascontiguousarray(b, np.float64)
+        host = np.zeros_like(a)
+        lib.oracle_math_probe(fn, rb.ptr(a, rb.c_d), rb.ptr(bb, rb.c_d), rb.ctypes.c_int64(len(a)), rb.ptr(host, rb.c_d))
+        dev = ctx.math_probe(fn, a, bb)
+        assert np.array_equal(dev.view(np.uint64), host.view(np.uint64)), fn
+    # IEEE division and square root on the device
+    a = rng.uniform(-1e3, 1e3, 5000).astype(np.float32).astype(np.float64)
+    b = rng.uniform(1e-3, 1e3, 5000).astype(np.float32).astype(np.float64)
+    dev = ctx.math_probe(7, a, b)
+    assert np.array_equal(dev.astype(np.float32), (a.astype(np.float32) / b.astype(np.float32)))
+    dev = ctx.math_probe(8, a, b)
+    assert np.array_equal(dev, a / b)
+    dev = ctx.math_probe(9, np.abs(a))
+    assert np.array_equal(dev.astype(np.float32), np.sqrt(np.abs(a).astype(np.float32)))
+
+
+def _vacuum_rays(g, n, seed):
+    rng = np.random.default_rng(seed)
+    v = np.zeros((n, 7), np.float32)
+    v[:, 0] = 100.0
+    lo = np.array([g.xe[0], g.ye[0], g.ze[0]])
+    hi = np.array([g.xe[-1], g.ye[-1], g.ze[-1]])
+    k = n // 2
+    # rays starting inside the grid, some on edges
+    p = rng.uniform(lo, hi, (k, 3))
+    snap = rng.random((k, 3)) < 0.3
+    p[snap[:, 0], 0] = g.xe[rng.integers(0, len(g.xe), snap[:, 0].sum())]
+    p[snap[:, 2], 2] = g.ze[rng.integers(0, len(g.ze), snap[:, 2].sum())]
+    # rays starting outside, aimed through the box
+    q = rng.uniform(lo - 30, hi + 30, (n - k, 3))
+    v[:k, 1:4] = p
+    v[k:, 1:4] = q
+    d = rng.normal(size=(n, 3))
+    axis = rng.random(n) < 0.2
+    d[axis] = np.eye(3)[rng.integers(0, 3, axis.sum())] * rng.choice([-1, 1], (axis.sum(), 1))
+    d[k:] = (0.5 * (lo + hi) - q) + rng.normal(scale=5, size=(n - k, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    v[:, 4:7] = d
+    return v
+
+
+def _split_ref(ref_cnb, ref_rec, n):
+    hist = ref_rec[:, 12].astype(np.int64)
+    return [(ref_cnb[hist == h], ref_rec[hist == h][:, :12]) for h in range(n)]
+
+
+def _assert_trace_equal(cnb, rec, nst, ref_split):
+    for h, (rc, rr) in enumerate(ref_split):
+        n = nst[h]
+        assert n == len(rc), (h, n, len(rc))
+        assert np.array_equal(cnb[h, :n], rc), h
+        assert np.array_equal(rec[h, :n].view(np.uint32), rr.view(np.uint32)), h
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_vacuum_traversal_bit_exact_vs_reference(ctx, exact):
+    """Deterministic ray traversal: voxel sequence, exit distances and the
+    in-place direction edits of grid3d::intersect, bit-exact against the
+    compiled reference transport on a vacuum grid (production and
+    verification kernels)."""
+    g = phantoms.water_box(20, 24, 30, dx=2.5, dz=3.0, rho=0.0)
+    rays = _vacuum_rays(g, 400, 11)
+    ref = _split_ref(*rb.ref_trace(1, g, rays), len(rays))
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_scorers(rb.SC_DOSE)
+    cfg = ctx.config(seed=5, exact=exact, batched=False)
+    cnb, rec, nst = ctx.trace(cfg, len(rays), 200, vertices=rays)
+    _assert_trace_equal(cnb, rec, nst, ref)
+
+
+def _oracle_trace_philox(g, vtx, debug):
+    cnb, rec = rb.oracle_trace(g, vtx, cap=400000, debug=debug, prec=32, rng=1, math=1)
+    return _split_ref(cnb, rec, len(vtx))
+
+
+@pytest.mark.parametrize("debug", [0, 1])
+def test_exact_physics_trace_matches_oracle(ctx, debug):
+    """Full physics, step by step: the verification kernels reproduce the CPU
+    restatement (Philox stream, portable math) bit for bit on every step."""
+    g = phantoms.slab_box(30, 30, 160)
+    rng = np.random.default_rng(7)
+    n = 48
+    v = np.zeros((n, 7), np.float32)
+    v[:, 0] = rng.uniform(60, 200, n)
+    v[:, 1:3] = rng.normal(scale=4, size=(n, 2))
+    v[:, 3] = 0.5
+    v[:, 6] = -1.0
+    ref = _oracle_trace_philox(g, v, debug)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_scorers(rb.SC_DOSE)
+    cfg = ctx.config(seed=0, exact=True, batched=False, physics_debug=bool(debug))
+    cnb, rec, nst = ctx.trace(cfg, n, 6000, vertices=v)
+    _assert_trace_equal(cnb, rec, nst, ref)
+
+
+@pytest.mark.parametrize("debug,batched", [(0, True), (1, True), (0, False)])
+def test_exact_dose_letd_matches_oracle(ctx, debug, batched):
+    """Spot sampling + transport + scoring on the slab phantom: doses and
+    LETd equal the oracle's up to FP64 summation order."""
+    g = phantoms.slab_box(30, 30, 200)
+    sp = phantoms.spots([120.0, 180.0, 90.0], 300, sigma=4.0, sigma_e_frac=0.01,
+                        positions=[(0, 0, 0.5), (5, -3, 0.5), (-6, 4, 0.5)])
+    mask = rb.SC_DOSE | rb.SC_LETD
+    ref, ntr = rb.oracle_run_spots(g, sp, mask=mask, kernel=2, prec=32, rng=1, math=1, debug=debug, master_seed=42,
+                                   threads=4)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(mask)
+    res = ctx.run(ctx.config(seed=42, exact=True, batched=batched, physics_debug=bool(debug)))
+    assert res.histories == ntr == 900
+    for k in range(3):
+        got = ctx.scorer(k)
+        want = ref[k]
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got), fin)
+        assert np.allclose(got[fin], want[fin], rtol=1e-9, atol=0), k
+
+
+def test_fresh_context_first_run_binds_work_list():
+    """Regression: the work-list arrays are (re)allocated by the run itself;
+    the launch must bind them after that, also on a context's first run."""
+    from moquimc_amd import _abi
+    c = _abi.Context(0)
+    try:
+        g = phantoms.slab_box(30, 30, 200)
+        sp = phantoms.spots([120.0, 180.0, 90.0], 300, sigma=4.0, sigma_e_frac=0.01,
+                            positions=[(0, 0, 0.5), (5, -3, 0.5), (-6, 4, 0.5)])
+        ref, _ = rb.oracle_run_spots(g, sp, mask=rb.SC_DOSE, kernel=2, prec=32, rng=1, math=1, debug=0,
+                                     master_seed=42, threads=4)
+        c.set_grid(g.xe, g.ye, g.ze, g.density)
+        c.set_spots(sp)
+        c.set_scorers(rb.SC_DOSE)
+        for batched in (False, True):
+            c.zero_scorers()
+            res = c.run(c.config(seed=42, exact=True, batched=batched))
+            assert res.histories == 900
+            assert np.allclose(c.scorer(0), ref[0], rtol=1e-9, atol=0)
+    finally:
+        c.close()
