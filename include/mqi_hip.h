@@ -70,6 +70,7 @@ typedef struct mqi_run_config {
 typedef struct mqi_run_result {
     uint64_t histories;   /* histories transported (tracked_particles) */
     uint64_t dropped;     /* secondaries dropped by full stacks (the reference drops them silently) */
+    uint64_t steps;       /* voxel steps taken by all tracks */
     uint32_t status;      /* bit 0: a Dij table overflowed; bit 1: a track hit the step cap (non-finite state) */
     int32_t  rounds;      /* statistical rounds run */
     double   kernel_ms;   /* device time of the transport launches (HIP events) */

@@ -32,7 +32,8 @@ class RunConfig(ctypes.Structure):
 
 class RunResult(ctypes.Structure):
     _fields_ = [
-        ("histories", ctypes.c_uint64), ("dropped", ctypes.c_uint64), ("status", ctypes.c_uint32),
+        ("histories", ctypes.c_uint64), ("dropped", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+        ("status", ctypes.c_uint32),
         ("rounds", ctypes.c_int32), ("kernel_ms", ctypes.c_double), ("stat", ctypes.c_double)]
 
 
@@ -42,6 +43,15 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch bundles its own libamdhip64
+        # (soname libamdhip64.so.7, loaded by path from torch/lib).  Loading
+        # it first makes this library's libamdhip64.so.7 dependency resolve to
+        # that same runtime; loading ours first would leave two runtimes in
+        # the process and torch would see no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libmqi_hip.so is not built (python -m moquimc_amd.build): " + LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)

@@ -270,6 +270,7 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.next          = c->d_counters;
     P.tracked       = c->d_counters + 1;
     P.drops         = c->d_counters + 2;
+    P.steps         = c->d_counters + 3;
     P.status        = c->d_status;
     P.hist_offset   = c->d_hoff;
     P.seed          = cfg.seed;
@@ -307,7 +308,7 @@ int
 launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
        int threads) {
     bind_buffers(c, P);
-    if (!P.density || !P.xe || !P.ye || !P.ze || !P.stack || !P.next || !P.tracked || !P.drops || !P.status ||
+    if (!P.density || !P.xe || !P.ye || !P.ze || !P.stack || !P.next || !P.tracked || !P.drops || !P.steps || !P.status ||
         !P.hist_offset || (!P.vertices && (!P.spots || !P.item_end || !P.item_spot || !P.item_scenario)))
         return (int) hipErrorInvalidValue;
     for (int s = 0; s < P.n_scorers; ++s)
@@ -685,7 +686,7 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
         if (out.stat <= cfg.stat_criterion || rounds >= max_rounds) break;
         if ((uint64_t) (rounds + 1) * round_len + cfg.hist_offset >= (1ull << 32)) break;
     }
-    unsigned long long cnt[3] = { 0, 0, 0 };
+    unsigned long long cnt[4] = { 0, 0, 0, 0 };
     unsigned int       st     = 0;
     HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost));
@@ -699,6 +700,7 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     }
     out.histories = cnt[1];
     out.dropped   = cnt[2];
+    out.steps     = cnt[3];
     out.status    = st;
     out.rounds    = rounds;
     out.kernel_ms = ms;
@@ -759,7 +761,7 @@ mqi_run_vertices(mqi_ctx* c, const float* vtx, uint64_t n, const uint32_t* spot_
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float t = 0;
     HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
-    unsigned long long cnt[3] = { 0, 0, 0 };
+    unsigned long long cnt[4] = { 0, 0, 0, 0 };
     HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
     hipFree(d_v);
     if (d_s) hipFree(d_s);
@@ -767,6 +769,7 @@ mqi_run_vertices(mqi_ctx* c, const float* vtx, uint64_t n, const uint32_t* spot_
         std::memset(res, 0, sizeof *res);
         res->histories = cnt[1];
         res->dropped   = cnt[2];
+        res->steps     = cnt[3];
         res->rounds    = 1;
         res->kernel_ms = t;
         res->stat      = -1.0;

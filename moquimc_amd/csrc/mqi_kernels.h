@@ -69,6 +69,7 @@ struct transport_params {
     // counters
     unsigned long long* tracked;
     unsigned long long* drops;
+    unsigned long long* steps;   // voxel steps taken (all tracks)
     unsigned int*       status;
     // per-step trace (verification)
     int64_t* trace_cnb;

@@ -695,7 +695,7 @@ __global__ void __launch_bounds__(256) k_transport(transport_params P) {
     L.scale    = 1.0f;
     L.scen_off = 0;
     c.g.sx = c.g.sy = c.g.sz = 0.0f;
-    unsigned long long done = 0;
+    unsigned long long done = 0, nstep = 0;
     uint32_t           scen = 0;
     const uint64_t     hoff = P.hist_offset ? *P.hist_offset : 0;
 
@@ -762,8 +762,10 @@ __global__ void __launch_bounds__(256) k_transport(transport_params P) {
             if (!L.active) continue;
         }
         step<DBG, TRACE>(P, c, L);
+        ++nstep;
     }
     atomicAdd(P.tracked, done);
+    atomicAdd(P.steps, nstep);
     if (c.drops) atomicAdd(P.drops, (unsigned long long) c.drops);
 }
 

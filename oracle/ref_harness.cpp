@@ -22,6 +22,7 @@
 // under the first table entry; it then reads r_steps[65535] (out of bounds,
 // undefined behaviour, a segfault on the CPU build).  Everywhere the reference
 // is well defined the specialisation computes the same value.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <thread>
@@ -372,7 +373,7 @@ ref_transport(int nx, int ny, int nz, const float* xe, const float* ye, const fl
               const float* density, const float* rot9, const float* t3, int64_t n_hist,
               const float* vtx, const int32_t* seeds, const uint32_t* spot_ids, int mask, int kernel,
               int n_threads, double** out, uint64_t* tracked, int64_t* log_cnb, float* log_rec,
-              int64_t log_cap, int64_t* log_n) {
+              int64_t log_cap, int64_t* log_n, double* seconds) {
     if (n_threads < 1) n_threads = 1;
     if (mask & SC_LOG) n_threads = 1;
     auto*          geo  = make_grid(nx, ny, nz, xe, ye, ze, density, rot9, t3);
@@ -413,10 +414,12 @@ ref_transport(int nx, int ny, int nz, const float* xe, const float* ye, const fl
         counts[t] = tracked_local;
         g_log     = nullptr;
     };
+    auto                     t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> pool;
     for (int t = 1; t < n_threads; ++t) pool.emplace_back(work, t);
     work(0);
     for (auto& th : pool) th.join();
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
     uint64_t tot = 0;
     for (auto c : counts) tot += c;

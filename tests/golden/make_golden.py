@@ -1,0 +1,200 @@
+"""Write the golden fixtures of tests/golden from the reference built from its
+own sources (oracle/_ref/libmoqui_ref_{debug,release}.so, see oracle/Makefile).
+
+    make -C oracle && python tests/golden/make_golden.py [--stats]
+
+Fixtures (inputs and the reference's outputs, nothing else):
+  reference_tables.npz      the seven tables as compiled
+  reference_functions.npz   hu_to_density, spr_default, radiation_length_default,
+                            cross sections / dEdx / relativistic quantities,
+                            update_post_vertex_direction on seeded inputs
+  reference_grid.npz        grid3d index / intersect / index-update probes
+  reference_trace.npz       per-step vacuum ray traversal of the transport kernel
+  reference_transport.npz   dose / LETd / stat scorers of short seeded runs
+                            (vertices and seeds included), debug and release
+  reference_profiles.npz    (--stats) depth-dose and lateral profiles of
+                            high-statistics runs for the statistical GPU tests
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import phantoms  # noqa: E402
+import refbind as rb  # noqa: E402
+
+
+def grid_arrays():
+    xe = rb.uniform_edges(-10, 10, 20)
+    ye = np.cumsum(np.r_[-7.0, np.full(12, 1.2)]).astype(np.float32)
+    ze = rb.uniform_edges(-30, 0, 15)
+    return xe, ye, ze
+
+
+def functions():
+    rng = np.random.default_rng(20)
+    out = {}
+    hu = np.arange(-1200, 3200, dtype=np.int16)
+    a = np.zeros(hu.size, np.float32)
+    rb.reference(0).ref_hu_to_density(rb.ptr(hu, rb.ctypes.c_void_p), rb.ctypes.c_int64(hu.size), rb.ptr(a, rb.c_f))
+    out["hu"], out["hu_density"] = hu, a
+    rho = np.concatenate([rng.uniform(1e-7, 3e-3, 2000), [1e-3, 0.9995e-3, 1.0009e-3, 0.26e-3, 0.9e-3]])
+    rho = rho.astype(np.float32)
+    ek = rng.uniform(0.0, 300.0, rho.size).astype(np.float32)
+    out["rho"], out["ek"] = rho, ek
+    for dbg in (0, 1):
+        x = np.zeros_like(rho)
+        rb.reference(dbg).ref_spr(rb.ptr(rho, rb.c_f), rb.ptr(ek, rb.c_f), rb.ctypes.c_int64(rho.size),
+                                  rb.ptr(x, rb.c_f))
+        out["spr_%d" % dbg] = x
+        y = np.zeros_like(rho)
+        rb.reference(dbg).ref_radiation_length(rb.ptr(rho, rb.c_f), rb.ctypes.c_int64(rho.size), rb.ptr(y, rb.c_f))
+        out["x0_%d" % dbg] = y
+    e = np.concatenate([np.linspace(0.0, 299.5, 1500), rng.uniform(0.05, 299, 1000)]).astype(np.float32)
+    out["probe_e"] = e
+    for r in (1e-3, 1.85e-3):
+        x = np.zeros((e.size, 10), np.float32)
+        rb.reference(0).ref_physics_probe(rb.ptr(e, rb.c_f), rb.ctypes.c_int64(e.size), rb.ctypes.c_float(r),
+                                          rb.ptr(x, rb.c_f))
+        out["probe_%g" % r] = x
+    dirs = rng.normal(size=(2000, 3)).astype(np.float32)
+    dirs[:100] = [0, 0, 1]
+    dirs[100:200] = [0, 0, -1]
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True).astype(np.float32)
+    th = rng.uniform(0, 0.5, 2000).astype(np.float32)
+    th[::7] = 0
+    ph = rng.uniform(0, 2 * np.pi, 2000).astype(np.float32)
+    ph[::11] = 0
+    x = np.zeros_like(dirs)
+    rb.reference(0).ref_rotate_direction(rb.ptr(dirs, rb.c_f), rb.ptr(th, rb.c_f), rb.ptr(ph, rb.c_f),
+                                         rb.ctypes.c_int64(2000), rb.ptr(x, rb.c_f))
+    out["rot_dir"], out["rot_theta"], out["rot_phi"], out["rot_out"] = dirs, th, ph, x
+    return out
+
+
+def grid_probes():
+    xe, ye, ze = grid_arrays()
+    rng = np.random.default_rng(21)
+    n = 3000
+    p = rng.uniform([-12, -8, -32], [12, 8, 2], (n, 3)).astype(np.float32)
+    snap = rng.random(n) < 0.3
+    p[snap, 0] = xe[rng.integers(0, len(xe), snap.sum())] + rng.choice([0, 5e-4, -5e-4, 2e-3], snap.sum())
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d[rng.random(n) < 0.1, 1] = 1e-5
+    d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+    cells = np.stack([rng.integers(0, 20, n), rng.integers(0, 12, n), rng.integers(0, 15, n)], 1).astype(np.int32)
+    lo = np.stack([xe[cells[:, 0]], ye[cells[:, 1]], ze[cells[:, 2]]], 1)
+    hi = np.stack([xe[cells[:, 0] + 1], ye[cells[:, 1] + 1], ze[cells[:, 2] + 1]], 1)
+    inside = (lo + rng.uniform(0, 1, (n, 3)) * (hi - lo)).astype(np.float32)
+    on = rng.random((n, 3)) < 0.2
+    inside[on] = hi[on]
+    out = {"xe": xe, "ye": ye, "ze": ze, "p": p, "p_inside": inside, "d": d, "cells": cells}
+    for op in range(4):
+        pp = inside if op == 1 else p
+        dist = np.zeros(n, np.float32)
+        dd = np.zeros((n, 3), np.float32)
+        cc = np.zeros((n, 3), np.int32)
+        rb.reference(0).ref_grid_probe(op, 20, 12, 15, rb.ptr(xe, rb.c_f), rb.ptr(ye, rb.c_f), rb.ptr(ze, rb.c_f),
+                                       rb.ptr(np.ascontiguousarray(pp), rb.c_f), rb.ptr(d, rb.c_f),
+                                       rb.ptr(cells, rb.c_i32), rb.ctypes.c_int64(n), rb.ptr(dist, rb.c_f),
+                                       rb.ptr(dd, rb.c_f), rb.ptr(cc, rb.c_i32))
+        out["op%d_dist" % op], out["op%d_dir" % op], out["op%d_cell" % op] = dist, dd, cc
+    return out
+
+
+def trace_fixture():
+    g = phantoms.water_box(16, 18, 20, dx=2.0, dz=2.5, rho=0.0)
+    rng = np.random.default_rng(22)
+    n = 300
+    v = np.zeros((n, 7), np.float32)
+    v[:, 0] = 50
+    lo = np.array([g.xe[0], g.ye[0], g.ze[0]])
+    hi = np.array([g.xe[-1], g.ye[-1], g.ze[-1]])
+    v[:, 1:4] = rng.uniform(lo - 10, hi + 10, (n, 3))
+    d = (0.5 * (lo + hi) - v[:, 1:4]) + rng.normal(scale=4, size=(n, 3))
+    v[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    cnb, rec = rb.ref_trace(1, g, v)
+    return {"xe": g.xe, "ye": g.ye, "ze": g.ze, "rays": v, "cnb": cnb, "rec": rec}
+
+
+def transport_fixture():
+    out = {}
+    g = phantoms.slab_box(10, 10, 160)
+    out["xe"], out["ye"], out["ze"], out["density"] = g.xe, g.ye, g.ze, g.density
+    for dbg in (1, 0):
+        sp = rb.make_spot(130.0, 2.0, 100, pos=(0, 0, 0.5), energy_const=False, sigma_e=1.3)
+        vtx, seeds = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], 5, 2, 100, debug=dbg)
+        out["vtx_%d" % dbg], out["seeds_%d" % dbg] = vtx, seeds
+        ra, _ = rb.ref_transport(dbg, g, vtx, seeds, mask=rb.SC_DOSE | rb.SC_LETD, kernel=0)
+        for k, a in enumerate(ra):
+            out["plain_%d_%d" % (dbg, k)] = a
+        ra, _ = rb.ref_transport(dbg, g, vtx, seeds, mask=rb.SC_DOSE | rb.SC_STAT, kernel=1)
+        for k, a in enumerate(ra):
+            out["stat_%d_%d" % (dbg, k)] = a
+    return out
+
+
+def profiles_fixture(threads):
+    """High-statistics reference runs: water (C1 geometry, debug physics) and
+    one slab spot (release), summarised as depth-dose / lateral profiles with
+    batch standard errors (8 independent seeds)."""
+    out = {}
+    cases = [
+        ("water", phantoms.water_box(100, 100, 350, dx=0.5, dz=1.0), rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5)),
+         1, 250000),
+        ("slab", phantoms.slab_box(80, 80, 300, dx=1.0, dz=1.0),
+         rb.make_spot(150.0, 4.0, 0, pos=(0, 0, 0.5), energy_const=False, sigma_e=1.5), 0, 250000),
+    ]
+    for name, g, sp, dbg, per_batch in cases:
+        dd, lat = [], []
+        t0 = time.time()
+        for b in range(8):
+            vtx, seeds = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], 100 + b, 0, per_batch, debug=dbg)
+            (dose,), tracked = rb.ref_transport(dbg, g, vtx, seeds, mask=rb.SC_DOSE, kernel=0, threads=threads)
+            d3 = dose.reshape(g.nz, g.ny, g.nx)
+            dd.append(d3.sum(axis=(1, 2)) / per_batch)
+            iz = [g.nz - 1 - k for k in (20, 100, 150)]
+            lat.append(np.stack([d3[z, g.ny // 2, :] + d3[z, g.ny // 2 - 1, :] for z in iz]) / per_batch)
+        dd = np.array(dd)
+        lat = np.array(lat)
+        out[name + "_xe"], out[name + "_ye"], out[name + "_ze"] = g.xe, g.ye, g.ze
+        out[name + "_density"] = g.density[:: g.nx * g.ny]
+        out[name + "_spot"] = np.array([sp], rb.SPOT_DTYPE)
+        out[name + "_debug"] = np.array(dbg)
+        out[name + "_histories"] = np.array(8 * per_batch)
+        out[name + "_depth_dose"] = dd.mean(0)
+        out[name + "_depth_dose_se"] = dd.std(0, ddof=1) / np.sqrt(8)
+        out[name + "_lateral"] = lat.mean(0)
+        out[name + "_lateral_se"] = lat.std(0, ddof=1) / np.sqrt(8)
+        print(name, "done in %.0f s" % (time.time() - t0), flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--stats", action="store_true")
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
+    args = ap.parse_args()
+    if not rb.have_reference():
+        raise SystemExit("build the reference harness first: make -C oracle")
+    tables = [np.zeros(600, np.float32) for _ in range(6)] + [np.zeros(3996, np.float32)]
+    rb.reference(0).ref_tables(*[rb.ptr(t, rb.c_f) for t in tables])
+    names = ["cs_p_ion", "dedx_restricted", "range_steps", "cs_pp_e", "cs_po_e", "cs_po_i", "density_correction"]
+    np.savez_compressed(os.path.join(HERE, "reference_tables.npz"), **dict(zip(names, tables)))
+    np.savez_compressed(os.path.join(HERE, "reference_functions.npz"), **functions())
+    np.savez_compressed(os.path.join(HERE, "reference_grid.npz"), **grid_probes())
+    np.savez_compressed(os.path.join(HERE, "reference_trace.npz"), **trace_fixture())
+    np.savez_compressed(os.path.join(HERE, "reference_transport.npz"), **transport_fixture())
+    if args.stats:
+        np.savez_compressed(os.path.join(HERE, "reference_profiles.npz"), **profiles_fixture(args.threads))
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -99,7 +99,8 @@ def _outs(mask, nvox):
     return arrs, table
 
 
-def ref_transport(debug, grid, vtx, seeds, spot_ids=None, mask=SC_DOSE, kernel=0, threads=1):
+def ref_transport(debug, grid, vtx, seeds, spot_ids=None, mask=SC_DOSE, kernel=0, threads=1, timing=None):
+    """Reference CPU transport; timing (a list) receives the seconds of the transport phase."""
     lib = reference(debug)
     lib.ref_transport.restype = ctypes.c_int
     vtx = np.ascontiguousarray(vtx, np.float32)
@@ -107,11 +108,15 @@ def ref_transport(debug, grid, vtx, seeds, spot_ids=None, mask=SC_DOSE, kernel=0
     n = len(seeds)
     arrs, table = _outs(mask, grid.nvox)
     tracked = np.zeros(1, np.uint64)
+    secs = np.zeros(1, np.float64)
     lib.ref_transport(grid.nx, grid.ny, grid.nz, ptr(grid.xe, c_f), ptr(grid.ye, c_f), ptr(grid.ze, c_f),
                       ptr(grid.density, c_f), ptr(grid.rot9, c_f), ptr(grid.t3, c_f), ctypes.c_int64(n),
                       ptr(vtx, c_f), ptr(seeds, c_i32),
                       ptr(None if spot_ids is None else np.ascontiguousarray(spot_ids, np.uint32), c_u32),
-                      mask, kernel, threads, table, ptr(tracked, c_u64), None, None, ctypes.c_int64(0), None)
+                      mask, kernel, threads, table, ptr(tracked, c_u64), None, None, ctypes.c_int64(0), None,
+                      ptr(secs, c_d))
+    if timing is not None:
+        timing.append(float(secs[0]))
     return arrs, int(tracked[0])
 
 
@@ -128,7 +133,7 @@ def ref_trace(debug, grid, vtx, cap=200000):
     lib.ref_transport(grid.nx, grid.ny, grid.nz, ptr(grid.xe, c_f), ptr(grid.ye, c_f), ptr(grid.ze, c_f),
                       ptr(grid.density, c_f), ptr(grid.rot9, c_f), ptr(grid.t3, c_f), ctypes.c_int64(n),
                       ptr(vtx, c_f), ptr(seeds, c_i32), None, SC_LOG, 0, 1, None, None,
-                      ptr(cnb, c_i64), ptr(rec, c_f), ctypes.c_int64(cap), ptr(nlog, c_i64))
+                      ptr(cnb, c_i64), ptr(rec, c_f), ctypes.c_int64(cap), ptr(nlog, c_i64), None)
     k = min(int(nlog[0]), cap)
     return cnb[:k], rec[:k]
 

@@ -1,0 +1,113 @@
+"""The C ABI on a machine without a GPU: the library loads, exports every
+function include/mqi_hip.h declares, the ctypes mirrors of the ABI structs
+have the C layout (checked against a program compiled from the header), and
+a context on a missing device fails with an error code and a message instead
+of exiting.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mqi_hip.h")
+LIB = os.path.join(ROOT, "moquimc_amd", "libmqi_hip.so")
+
+needs_lib = pytest.mark.skipif(not os.path.exists(LIB), reason="libmqi_hip.so not built (python -m moquimc_amd.build)")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mqi_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    names = _declared()
+    for must in ("mqi_create", "mqi_destroy", "mqi_last_error", "mqi_run", "mqi_dij_matvec"):
+        assert must in names
+    from moquimc_amd import _abi
+    assert sorted(_abi.EXPORTED) == names
+
+
+@needs_lib
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [n for n in _declared() if n not in exported]
+    assert not missing, missing
+    lib = ctypes.CDLL(LIB)
+    for n in _declared():
+        assert getattr(lib, n) is not None
+
+
+def _c_layout():
+    """sizeof / offsetof of the ABI structs from a host program compiled
+    against the header with gcc."""
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "mqi_hip.h"
+#define F(S, m) printf(#S "." #m " %zu\n", offsetof(S, m));
+int main(void) {
+    printf("mqi_spot %zu\nmqi_run_config %zu\nmqi_run_result %zu\n", sizeof(mqi_spot), sizeof(mqi_run_config),
+           sizeof(mqi_run_result));
+    F(mqi_spot, mean) F(mqi_spot, sigma) F(mqi_spot, corr) F(mqi_spot, rotation) F(mqi_spot, translation)
+    F(mqi_spot, histories)
+    F(mqi_run_config, stat_threshold) F(mqi_run_config, max_rounds) F(mqi_run_config, dij)
+    F(mqi_run_config, dij_capacity) F(mqi_run_config, hist_offset)
+    F(mqi_run_result, dropped) F(mqi_run_result, steps) F(mqi_run_result, status) F(mqi_run_result, kernel_ms)
+    F(mqi_run_result, stat)
+    return 0;
+}
+"""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    return {k: int(v) for k, v in (ln.split() for ln in out.splitlines())}
+
+
+def test_ctypes_mirrors_match_c_layout():
+    from moquimc_amd import _abi
+    lay = _c_layout()
+    assert lay["mqi_spot"] == _abi.SPOT_DTYPE.itemsize == 128
+    for f in ("mean", "sigma", "corr", "rotation", "translation", "histories"):
+        assert lay["mqi_spot." + f] == _abi.SPOT_DTYPE.fields[f][1], f
+    assert lay["mqi_run_config"] == ctypes.sizeof(_abi.RunConfig)
+    assert lay["mqi_run_result"] == ctypes.sizeof(_abi.RunResult)
+    for f in ("stat_threshold", "max_rounds", "dij", "dij_capacity", "hist_offset"):
+        assert lay["mqi_run_config." + f] == getattr(_abi.RunConfig, f).offset, f
+    for f in ("dropped", "steps", "status", "kernel_ms", "stat"):
+        assert lay["mqi_run_result." + f] == getattr(_abi.RunResult, f).offset, f
+
+
+@needs_lib
+def test_create_without_device_reports_error():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = ctypes.CDLL(LIB)
+    lib.mqi_last_error.restype = ctypes.c_char_p
+    lib.mqi_last_error.argtypes = [ctypes.c_void_p]
+    h = ctypes.c_void_p()
+    rc = lib.mqi_create(0, ctypes.byref(h))
+    assert rc != 0
+    assert not h.value
+    msg = lib.mqi_last_error(None)
+    assert msg and len(msg) > 0
+
+
+def test_null_context_calls_fail_cleanly():
+    if not os.path.exists(LIB):
+        pytest.skip("libmqi_hip.so not built")
+    lib = ctypes.CDLL(LIB)
+    assert lib.mqi_destroy(None) in (0, 1)
+    assert lib.mqi_zero_scorers(None) == 1
+    assert lib.mqi_run(None, None, None) == 1
