@@ -117,6 +117,19 @@ def adjoint_bench(ctx_cls, reps=20):
                                  "kernel": "k_csr_rows (D^T g)", "algorithmic_bytes": int(alg_bytes)}}
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its
+    dispatches against nbytes give the scale of the HBM counters."""
+    import torch
+    dev = torch.device("cuda", 0)
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,11 +139,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--exact", action="store_true", help="time the verification kernels instead")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (the reference kernel of the HBM counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take its own slice "
               "of the (scenario, spot) work list; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from moquimc_amd import _abi, workloads
     t0 = time.time()

@@ -60,7 +60,8 @@ struct mqi_ctx {
     double*          d_out[MQI_MAX_SCORERS] = { nullptr };
 
     // counters: next, tracked, drops
-    unsigned long long* d_counters = nullptr;
+    unsigned long long* d_counters = nullptr;   // [1] tracked, [2] dropped, [3] steps
+    unsigned long long* d_queue    = nullptr;   // per-XCD work-range counters of dynamic launches
     unsigned int*       d_status   = nullptr;
     uint64_t*           d_hoff     = nullptr;
     float*              d_statmax  = nullptr;
@@ -267,7 +268,8 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.item_end      = c->d_item_end;
     P.item_spot     = c->d_item_spot;
     P.item_scenario = c->d_item_scen;
-    P.next          = c->d_counters;
+    P.queue         = c->d_queue;
+    P.fetch_chunk   = cfg.fetch_chunk > 0 ? cfg.fetch_chunk : 1;
     P.tracked       = c->d_counters + 1;
     P.drops         = c->d_counters + 2;
     P.steps         = c->d_counters + 3;
@@ -308,7 +310,8 @@ int
 launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
        int threads) {
     bind_buffers(c, P);
-    if (!P.density || !P.xe || !P.ye || !P.ze || !P.stack || !P.next || !P.tracked || !P.drops || !P.steps || !P.status ||
+    if (P.fetch_chunk < 1) return (int) hipErrorInvalidValue;
+    if (!P.density || !P.xe || !P.ye || !P.ze || !P.stack || !P.queue || !P.tracked || !P.drops || !P.steps || !P.status ||
         !P.hist_offset || (!P.vertices && (!P.spots || !P.item_end || !P.item_spot || !P.item_scenario)))
         return (int) hipErrorInvalidValue;
     for (int s = 0; s < P.n_scorers; ++s)
@@ -317,16 +320,33 @@ launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, 
     if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * 9 * sizeof(float) > c->stack_bytes)
         return (int) hipErrorInvalidValue;
     const size_t sh = shmem_bytes(c);
-    if (cfg.exact) return mqi_exact::launch_transport(P, cfg.physics_debug, dynamic, trace, blocks, threads, sh, c->stream);
-    return mqi_fast::launch_transport(P, cfg.physics_debug, dynamic, trace, blocks, threads, sh, c->stream);
+    if (dynamic) {
+        const int e = mqi_aux::queue_init(c->d_queue, P.total, c->stream);
+        if (e) return e;
+    }
+    if (cfg.exact)
+        return mqi_exact::launch_transport(P, cfg.physics_debug, dynamic, trace, 0, blocks, threads, sh, c->stream);
+    return mqi_fast::launch_transport(P, cfg.physics_debug, dynamic, trace, cfg.waves_per_simd, blocks, threads, sh,
+                                      c->stream);
+}
+
+int
+check_tuning(mqi_ctx* c, const mqi_run_config& cfg) {
+    const int w = cfg.waves_per_simd;
+    if (w != 0 && (w < 2 || w > 6) && w != 8) return fail(c, MQI_E_ARG, "waves_per_simd must be 0, 2-6 or 8");
+    if (cfg.fetch_chunk < 0 || cfg.fetch_chunk > 4096) return fail(c, MQI_E_ARG, "fetch_chunk must be in [0, 4096]");
+    if (cfg.block_size != 0 && (cfg.block_size % 64 != 0 || cfg.block_size > 256 || cfg.block_size < 0))
+        return fail(c, MQI_E_ARG, "block_size must be 64, 128, 192 or 256");
+    if (cfg.blocks < 0) return fail(c, MQI_E_ARG, "blocks must be >= 0");
+    return 0;
 }
 
 int
 persistent_blocks(mqi_ctx* c, const mqi_run_config& cfg, int threads, int trace) {
     if (cfg.blocks > 0) return cfg.blocks;
     const size_t sh  = shmem_bytes(c);
-    int          occ = cfg.exact ? mqi_exact::occupancy(cfg.physics_debug, 1, trace, threads, sh)
-                                 : mqi_fast::occupancy(cfg.physics_debug, 1, trace, threads, sh);
+    int          occ = cfg.exact ? mqi_exact::occupancy(cfg.physics_debug, 1, trace, 0, threads, sh)
+                                 : mqi_fast::occupancy(cfg.physics_debug, 1, trace, cfg.waves_per_simd, threads, sh);
     return c->n_cu * occ;
 }
 
@@ -365,7 +385,6 @@ transport_pass(mqi_ctx* c, const mqi_run_config& cfg, uint64_t hoff, double* ms)
         const int blocks = persistent_blocks(c, cfg, threads, 0);
         if ((r = ensure_stack(c, lane_bytes * (size_t) blocks * threads))) return r;
         P.stack = c->d_stack;
-        KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 1, c->stream));
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         KCHK(c, launch(c, P, cfg, 1, 0, blocks, threads));
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -476,7 +495,7 @@ mqi_create(int device, mqi_ctx** out) {
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
     int r;
-    if ((r = dalloc(c, &c->d_counters, 4)) || (r = dalloc(c, &c->d_status, 1)) || (r = dalloc(c, &c->d_hoff, 1)) ||
+    if ((r = dalloc(c, &c->d_counters, 4)) || (r = dalloc(c, &c->d_queue, MQI_QUEUES * MQI_QUEUE_STRIDE)) || (r = dalloc(c, &c->d_status, 1)) || (r = dalloc(c, &c->d_hoff, 1)) ||
         (r = dalloc(c, &c->d_statmax, 1)) || (r = dalloc(c, &c->d_statsum, 1)) || (r = dalloc(c, &c->d_statcnt, 1))) {
         mqi_destroy(c);
         return r;
@@ -506,6 +525,7 @@ mqi_destroy(mqi_ctx* c) {
     dfree(c->d_scale);
     for (auto& p : c->d_out) dfree(p);
     dfree(c->d_counters);
+    dfree(c->d_queue);
     dfree(c->d_status);
     dfree(c->d_hoff);
     dfree(c->d_statmax);
@@ -662,6 +682,7 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     if (c->spots.empty()) return fail(c, MQI_E_STATE, "no spots: call mqi_set_spots first");
     hipSetDevice(c->device);
     mqi_run_config cfg = *cfg_in;
+    if ((r = check_tuning(c, cfg))) return r;
     if (cfg.scoring_loops < 0 || cfg.scoring_loops > 2) return fail(c, MQI_E_ARG, "bad scoring_loops");
     const int n_sc = (int) c->kinds.size() + (cfg.dij ? 1 : 0);
     if (n_sc == 0) return fail(c, MQI_E_STATE, "no scorers");
@@ -721,6 +742,7 @@ mqi_run_vertices(mqi_ctx* c, const float* vtx, uint64_t n, const uint32_t* spot_
     if (c->kinds.empty()) return fail(c, MQI_E_STATE, "no scorers");
     hipSetDevice(c->device);
     mqi_run_config cfg = *cfg_in;
+    if ((r = check_tuning(c, cfg))) return r;
     if (cfg.dij) return fail(c, MQI_E_ARG, "Dij scoring needs a spot table (mqi_run)");
     float*    d_v = nullptr;
     uint32_t* d_s = nullptr;
@@ -786,6 +808,7 @@ mqi_trace(mqi_ctx* c, const mqi_run_config* cfg_in, const float* vertices, uint6
         return fail(c, MQI_E_ARG, "bad trace arguments");
     hipSetDevice(c->device);
     mqi_run_config cfg = *cfg_in;
+    if ((r = check_tuning(c, cfg))) return r;
     run_plan       p;
     float*         d_v = nullptr;
     if (vertices) {

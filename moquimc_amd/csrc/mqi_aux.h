@@ -8,6 +8,8 @@
 namespace mqi_aux
 {
 int fill_u64(void* p, unsigned long long v, uint64_t n, hipStream_t s);
+// counters of the per-XCD work ranges of a dynamic launch over `total` histories
+int queue_init(unsigned long long* q, uint64_t total, hipStream_t s);
 int fill_u32(void* p, uint32_t v, uint64_t n, hipStream_t s);
 int hu_to_density(const int16_t* hu, float* rho, uint64_t n, float scale, hipStream_t s);
 int math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t s);

@@ -6,6 +6,7 @@
 
 #include "mqi_aux.h"
 #include "mqi_device.hpp"
+#include "mqi_kernels.h"
 
 #define MQI_TABLE_QUALIFIER static __constant__
 #include "mqi_tables.inc"
@@ -31,6 +32,17 @@ grid_for(uint64_t n) {
     if (b > 65536) b = 65536;
     if (b < 1) b = 1;
     return (int) b;
+}
+
+__global__ void k_queue_init(unsigned long long* q, uint64_t total) {
+    const int i = threadIdx.x;
+    if (i < MQI_QUEUES) q[i * MQI_QUEUE_STRIDE] = mqi_range_begin(total, i);
+}
+
+int
+queue_init(unsigned long long* q, uint64_t total, hipStream_t s) {
+    hipLaunchKernelGGL(k_queue_init, dim3(1), dim3(64), 0, s, q, total);
+    return (int) hipGetLastError();
 }
 
 int

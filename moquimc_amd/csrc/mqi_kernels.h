@@ -30,6 +30,17 @@ enum {
 #define MQI_STATUS_TRACK_CAP 2u
 #define MQI_MAX_TRACK_STEPS 200000
 
+// Work ranges of the dynamic form: one per XCD of the MI355X, counters a
+// 128-byte line apart.
+#define MQI_QUEUES 8
+#define MQI_QUEUE_STRIDE 16
+
+// first history of range q of a list of `total` histories (balanced split)
+__host__ __device__ inline uint64_t mqi_range_begin(uint64_t total, int q) {
+    const uint64_t base = total / MQI_QUEUES, rem = total % MQI_QUEUES;
+    return base * (uint64_t) q + ((uint64_t) q < rem ? (uint64_t) q : rem);
+}
+
 struct transport_params {
     // geometry (one voxel child of the world)
     int          nx, ny, nz;
@@ -51,7 +62,11 @@ struct transport_params {
     const uint32_t* item_scenario;
     int             n_items;
     uint64_t        total;
-    unsigned long long* next;   // dynamic fetch counter
+    // dynamic form: MQI_QUEUES contiguous ranges of the flattened work list,
+    // one per XCD, each with its own counter at queue[q * MQI_QUEUE_STRIDE]
+    // (initialised to the range's first history)
+    unsigned long long* queue;
+    int             fetch_chunk;   // histories a lane takes per fetch (dynamic form), >= 1
     const uint64_t* hist_offset;   // device scalar added to every history index (graph-replay safe)
     uint64_t        seed;
     // scorers
@@ -80,15 +95,15 @@ struct transport_params {
 
 namespace mqi_fast
 {
-int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int blocks, int threads,
-                     size_t shmem, hipStream_t stream);
-int occupancy(int debug, int dynamic, int trace, int threads, size_t shmem);
+int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int waves, int blocks,
+                     int threads, size_t shmem, hipStream_t stream);
+int occupancy(int debug, int dynamic, int trace, int waves, int threads, size_t shmem);
 }
 namespace mqi_exact
 {
-int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int blocks, int threads,
-                     size_t shmem, hipStream_t stream);
-int occupancy(int debug, int dynamic, int trace, int threads, size_t shmem);
+int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int waves, int blocks,
+                     int threads, size_t shmem, hipStream_t stream);
+int occupancy(int debug, int dynamic, int trace, int waves, int threads, size_t shmem);
 }
 
 #endif

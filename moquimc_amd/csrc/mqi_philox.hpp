@@ -61,7 +61,7 @@ struct stream {
     uint32_t key0, key1;
     uint32_t c1, c2, c3;
     uint32_t block;
-    uint32_t buf[4];
+    uint32_t q0, q1, q2, q3;   // unread words of the current block, next first
     int      nbuf;
     int      has_spare;
     double   spare;
@@ -79,6 +79,9 @@ struct stream {
         spare     = 0.0;
     }
 
+    // Words of a block are consumed in the order o[0], o[1], o[2], o[3].  The
+    // queue is shifted rather than indexed: a runtime index into an array
+    // member would put the stream in private memory on the GPU.
     MQI_HD uint32_t
     next_u32() {
         if (nbuf == 0) {
@@ -88,13 +91,18 @@ struct stream {
             c.v[2] = c2;
             c.v[3] = c3;
             u32x4 o = philox4x32_10(c, key0, key1);
-            buf[0]  = o.v[3];
-            buf[1]  = o.v[2];
-            buf[2]  = o.v[1];
-            buf[3]  = o.v[0];
+            q0      = o.v[0];
+            q1      = o.v[1];
+            q2      = o.v[2];
+            q3      = o.v[3];
             nbuf    = 4;
         }
-        return buf[--nbuf];
+        const uint32_t r = q0;
+        q0               = q1;
+        q1               = q2;
+        q2               = q3;
+        --nbuf;
+        return r;
     }
 
     MQI_HD float

@@ -40,8 +40,8 @@ struct ctx {
     float*       stack;   // HBM, stack_cap * 9 floats for this lane
     int          sp;
     int          cap;
-    const float* fwd;
-    const float* trans;
+    float        fwd[9];   // held by value: a pointer into the kernel
+    float        trans[3]; // arguments would force them into private memory
     unsigned     drops;
 };
 
@@ -434,8 +434,14 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
                   uint64_t dij_item, uint64_t scen_off) {
     const int     n   = P.n_scorers;
     const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT
-    double        v[MQI_MAX_SCORERS];
-    for (int s = 0; s < n; ++s) {
+    // Both loops run over the fixed maximum with constant indices so that the
+    // scorer table stays in kernel arguments / registers (a runtime index
+    // into it would move the whole argument block to private memory).
+    double v[MQI_MAX_SCORERS];
+#pragma unroll
+    for (int s = 0; s < MQI_MAX_SCORERS; ++s) {
+        v[s] = 0.0;
+        if (s >= n) continue;
         const int k = P.kinds[s];
         if (k == MQI_KIND_LETD_NUM)
             v[s] = letd_hit(t, density, true);
@@ -446,7 +452,9 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             v[s]     = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
         }
     }
-    for (int s = 0; s < n; ++s) {
+#pragma unroll
+    for (int s = 0; s < MQI_MAX_SCORERS; ++s) {
+        if (s >= n) continue;
         const int  k    = P.kinds[s];
         const bool stat = (k == MQI_KIND_STAT_DOSE || k == MQI_KIND_STAT_DOSE2);
         int        reps = 0;
@@ -660,8 +668,11 @@ MQI_DI void stage(const transport_params& P, float* sh) {
 // (batched persistent form); otherwise each thread takes a contiguous slice
 // [start, start+len) of the work list (plain form; start_and_length,
 // mqi_utils.hpp:138-147).
-template<bool DBG, bool DYNAMIC, bool TRACE>
-__global__ void __launch_bounds__(256) k_transport(transport_params P) {
+// W > 0 asks the register allocator for at least W resident waves per SIMD
+// (more memory latency hidden, at the price of spills past the budget).
+template<bool DBG, bool DYNAMIC, bool TRACE, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
+k_transport(transport_params P) {
     extern __shared__ float sh[];
     stage(P, sh);
     ctx c;
@@ -672,8 +683,10 @@ __global__ void __launch_bounds__(256) k_transport(transport_params P) {
     c.g.xe  = sh + T_TABLES;
     c.g.ye  = c.g.xe + P.nx + 1;
     c.g.ze  = c.g.ye + P.ny + 1;
-    c.fwd   = P.fwd;
-    c.trans = P.trans;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) c.fwd[i] = P.fwd[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c.trans[i] = P.trans[i];
     c.cap   = P.stack_cap;
     c.sp    = 0;
     c.drops = 0;
@@ -681,6 +694,8 @@ __global__ void __launch_bounds__(256) k_transport(transport_params P) {
     const uint64_t nthreads = (uint64_t) gridDim.x * blockDim.x;
     c.stack                 = P.stack + tid * (uint64_t) P.stack_cap * 9;
 
+    // plain: the thread's start_and_length slice; dynamic: the lane's current
+    // chunk of P.fetch_chunk consecutive histories from the device counter
     uint64_t s_next = 0, s_end = 0;
     if (!DYNAMIC) {   // start_and_length over the flattened list
         const uint64_t q = P.total / nthreads, r = P.total % nthreads;
@@ -698,20 +713,51 @@ __global__ void __launch_bounds__(256) k_transport(transport_params P) {
     unsigned long long done = 0, nstep = 0;
     uint32_t           scen = 0;
     const uint64_t     hoff = P.hist_offset ? *P.hist_offset : 0;
+    int                qx   = 0;   // work range this wave fetches from (dynamic form)
+    if (DYNAMIC) {
+        unsigned xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        qx = (int) (xcc & (MQI_QUEUES - 1));
+    }
 
     for (;;) {
         if (!L.active && c.sp == 0) {
             uint64_t g;
             if (DYNAMIC) {
-                const unsigned long long m    = __ballot(1);
-                const int                lead = __builtin_ctzll(m);
-                const int                me   = __lane_id();
-                const int                rank = __popcll(m & ((1ull << me) - 1ull));
-                unsigned long long       base = 0;
-                if (me == lead) base = atomicAdd(P.next, (unsigned long long) __popcll(m));
-                base = __shfl(base, lead);
-                g    = base + rank;
-                if (g >= P.total) break;
+                if (s_next >= s_end) {
+                    // Wave-aggregated fetch of fetch_chunk histories per dry
+                    // lane from range qx (initially this XCD's own range; a
+                    // range that runs dry sends the wave to the next one).
+                    const uint64_t chunk = (uint64_t) P.fetch_chunk;
+                    const int      me    = __lane_id();
+                    bool           got   = false;
+                    for (int r = 0; r < MQI_QUEUES; ++r) {
+                        const unsigned long long m = __ballot(!got);
+                        if (m == 0) break;
+                        const int          lead = __builtin_ctzll(m);
+                        const int          rank = __popcll(m & ((1ull << me) - 1ull));
+                        // the leader's range for every lane (lanes that were
+                        // busy during earlier fetches may hold a stale qx)
+                        const int          q    = __shfl(qx, lead);
+                        unsigned long long base = 0;
+                        if (me == lead)
+                            base = atomicAdd(P.queue + q * MQI_QUEUE_STRIDE, (unsigned long long) __popcll(m) * chunk);
+                        base               = __shfl(base, lead);
+                        const uint64_t end = mqi_range_begin(P.total, q + 1);
+                        if (!got) {
+                            const uint64_t s0 = base + rank * chunk;
+                            if (s0 < end) {
+                                s_next = s0;
+                                s_end  = s0 + chunk < end ? s0 + chunk : end;
+                                got    = true;
+                            }
+                        }
+                        // a range that could not serve every lane is dry: move on
+                        qx = (__ballot(!got) != 0) ? ((q + 1) & (MQI_QUEUES - 1)) : q;
+                    }
+                    if (!got) break;
+                }
+                g = s_next++;
             } else {
                 if (s_next >= s_end) break;
                 g = s_next++;
@@ -769,49 +815,70 @@ __global__ void __launch_bounds__(256) k_transport(transport_params P) {
     if (c.drops) atomicAdd(P.drops, (unsigned long long) c.drops);
 }
 
-int
-launch_transport(const transport_params& P, int debug, int dynamic, int trace, int blocks, int threads,
-                 size_t shmem, hipStream_t stream) {
-#define MQI_LAUNCH(D, Y, T)                                                                         \
-    hipLaunchKernelGGL((k_transport<D, Y, T>), dim3(blocks), dim3(threads), shmem, stream, P)
-    if (debug) {
-        if (dynamic) {
-            if (trace) MQI_LAUNCH(true, true, true); else MQI_LAUNCH(true, true, false);
-        } else {
-            if (trace) MQI_LAUNCH(true, false, true); else MQI_LAUNCH(true, false, false);
-        }
-    } else {
-        if (dynamic) {
-            if (trace) MQI_LAUNCH(false, true, true); else MQI_LAUNCH(false, true, false);
-        } else {
-            if (trace) MQI_LAUNCH(false, false, true); else MQI_LAUNCH(false, false, false);
-        }
+namespace
+{
+using launch_fn = void (*)(const transport_params&, int, int, size_t, hipStream_t);
+
+struct variant {
+    launch_fn   launch;
+    const void* fn;
+};
+
+template<bool D, bool Y, bool T, int W>
+void launch_one(const transport_params& P, int blocks, int threads, size_t shmem, hipStream_t stream) {
+    hipLaunchKernelGGL((k_transport<D, Y, T, W>), dim3(blocks), dim3(threads), shmem, stream, P);
+}
+
+template<bool D, bool Y, bool T, int W>
+variant make() {
+    return {&launch_one<D, Y, T, W>, reinterpret_cast<const void*>(&k_transport<D, Y, T, W>)};
+}
+
+template<bool D, bool Y, bool T>
+variant pick_waves(int waves) {
+#if MQI_EXACT
+    (void) waves;   // the verification kernels are not tuned
+    return make<D, Y, T, 0>();
+#else
+    if (T) return make<D, Y, T, 0>();
+    // 0: the measured optimum of the production kernels (4 waves/SIMD: the
+    // transport is latency-bound and past 4 the spills cost more than the
+    // extra waves hide); 2: the compiler's own register budget
+    switch (waves) {
+    case 2: return make<D, Y, T, 0>();
+    case 3: return make<D, Y, T, 3>();
+    case 5: return make<D, Y, T, 5>();
+    case 6: return make<D, Y, T, 6>();
+    case 8: return make<D, Y, T, 8>();
+    default: return make<D, Y, T, 4>();
     }
-#undef MQI_LAUNCH
+#endif
+}
+
+variant pick(int debug, int dynamic, int trace, int waves) {
+    if (debug) {
+        if (dynamic) return trace ? pick_waves<true, true, true>(waves) : pick_waves<true, true, false>(waves);
+        return trace ? pick_waves<true, false, true>(waves) : pick_waves<true, false, false>(waves);
+    }
+    if (dynamic) return trace ? pick_waves<false, true, true>(waves) : pick_waves<false, true, false>(waves);
+    return trace ? pick_waves<false, false, true>(waves) : pick_waves<false, false, false>(waves);
+}
+}   // namespace
+
+int
+launch_transport(const transport_params& P, int debug, int dynamic, int trace, int waves, int blocks, int threads,
+                 size_t shmem, hipStream_t stream) {
+    pick(debug, dynamic, trace, waves).launch(P, blocks, threads, shmem, stream);
     return (int) hipGetLastError();
 }
 
 // resident workgroups per CU of one instantiation
 int
-occupancy(int debug, int dynamic, int trace, int threads, size_t shmem) {
-    int         n = 0;
-    const void* f = nullptr;
-#define MQI_F(D, Y, T) f = reinterpret_cast<const void*>(&k_transport<D, Y, T>)
-    if (debug) {
-        if (dynamic) {
-            if (trace) MQI_F(true, true, true); else MQI_F(true, true, false);
-        } else {
-            if (trace) MQI_F(true, false, true); else MQI_F(true, false, false);
-        }
-    } else {
-        if (dynamic) {
-            if (trace) MQI_F(false, true, true); else MQI_F(false, true, false);
-        } else {
-            if (trace) MQI_F(false, false, true); else MQI_F(false, false, false);
-        }
-    }
-#undef MQI_F
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, shmem) != hipSuccess) return 1;
+occupancy(int debug, int dynamic, int trace, int waves, int threads, size_t shmem) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pick(debug, dynamic, trace, waves).fn, threads, shmem) !=
+        hipSuccess)
+        return 1;
     return n < 1 ? 1 : n;
 }
 
