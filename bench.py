@@ -108,13 +108,25 @@ def adjoint_bench(ctx_cls, reps=20):
     ms_med = float(np.median(ms))
     alg_bytes = 8.0 * nnz + 8.0 * (rows + 1) + 8.0 * rows + 4.0 * cols
     gbs = alg_bytes / (ms_med * 1e-3) / 1e9
+    # forward D w on the voxel-major copy (float32 dose out)
+    w8 = torch.rand(rows, device=dev, dtype=torch.float32) + 0.5
+    d = torch.zeros(cols, device=dev, dtype=torch.float32)
+    for _ in range(3):
+        ctx.dij_matvec(0, 0, w8.data_ptr(), d.data_ptr(), y_double=False)
+    fms = []
+    for _ in range(reps):
+        ctx.dij_matvec(0, 0, w8.data_ptr(), d.data_ptr(), y_double=False)
+        fms.append(ctx.last_matvec_ms())
+    f_med = float(np.median(fms))
+    f_bytes = 8.0 * nnz + 8.0 * (cols + 1) + 4.0 * cols + 4.0 * rows
     ctx.close()
     return {"dij_workload": w.name, "dij_histories": int(res.histories), "dij_transport_s": round(t_tr, 2),
             "dij_nnz": int(nnz), "dij_rows": int(rows), "dij_cols": int(cols),
             "adjoint_ms": round(ms_med, 4), "adjoint_GBs": round(gbs, 1),
+            "forward_ms": round(f_med, 4), "forward_GBs": round(f_bytes / (f_med * 1e-3) / 1e9, 1),
             "adjoint_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                                 "kernel": "k_csr_rows (D^T g)", "algorithmic_bytes": int(alg_bytes)}}
+                                 "kernel": "k_csr_adaptive + k_csr_split_rows (D^T g)", "algorithmic_bytes": int(alg_bytes)}}
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):

@@ -128,6 +128,17 @@ int mqi_scorer_count(mqi_ctx* ctx);
  * (mqi_tps_env.hpp:1040-1603) and phantom_env::run (mqi_phantom_env.hpp:303-428). */
 int mqi_run(mqi_ctx* ctx, const mqi_run_config* cfg, mqi_run_result* res);
 
+/* Autotuner (no counterpart in the reference, which fixes 512 threads per
+ * block and one history per thread, mqi_tps_env.hpp:1058-1083): times
+ * batched passes of the current plan with every spot's histories scaled by
+ * `sample` (0 < sample <= 1) over waves_per_simd {3, 4, 5} x block_size
+ * {128, 256} x fetch_chunk {1, 2, 4} and writes the fastest into *cfg.  With
+ * cfg->stack_depth < 0 it also picks the smallest secondary stack (16 .. 512)
+ * that drops nothing on the sample (the reference's depth drops silently,
+ * mqi_track_stack.hpp:36-41).  The trial passes score into the scorers, which
+ * are cleared at the end.  *best_ms: device time of the fastest trial. */
+int mqi_autotune(mqi_ctx* ctx, mqi_run_config* cfg, double sample, double* best_ms);
+
 /* Transport caller-made vertices (7 floats each: ke, x, y, z, u, v, w), the
  * upload_vertices + transport_particles_patient path.  spot_ids may be NULL
  * (the scorer_offset_vector, mqi_transport.hpp:150-154). */
@@ -167,11 +178,14 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
 /* y = alpha * A x (+ y if accumulate) on the device, A = D (voxels x spots,
  * transpose = 0, the forward dose D w) or D^T (transpose = 1, the adjoint
  * D^T g).  x: float32 device vector; y: float64 if y_double else float32;
- * FP64 accumulation.  stream: a hipStream_t or NULL for the context's stream. */
+ * FP64 accumulation in a fixed order (results do not depend on scheduling).
+ * stream: a hipStream_t or NULL for the context's stream.  The call only
+ * enqueues work: y is ready when that stream reaches it. */
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
 
-/* Device time (ms, HIP events) of the last mqi_dij_matvec, for measurement. */
+/* Device time (ms, HIP events) of the last mqi_dij_matvec, for measurement;
+ * waits for that product to finish. */
 int mqi_last_matvec_ms(mqi_ctx* ctx, double* ms);
 
 /* Verification: the portable math functions on the device (fn 0 log, 1 exp,

@@ -83,6 +83,7 @@ def lib():
             "mqi_dij_device_ptrs": (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
             "mqi_dij_matvec": (i, [vp, i, i, vp, vp, i, d, i, vp]),
             "mqi_last_matvec_ms": (i, [vp, ctypes.POINTER(d)]),
+            "mqi_autotune": (i, [vp, ctypes.POINTER(RunConfig), d, ctypes.POINTER(d)]),
             "mqi_math_probe": (i, [vp, i, vp, vp, ctypes.c_int64, vp]),
         }
         for name, (res, args) in sig.items():
@@ -97,7 +98,7 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_get_density", "mqi_set_spots", "mqi_set_scenarios", "mqi_set_scorers", "mqi_zero_scorers",
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
-            "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe"]
+            "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune"]
 
 
 class MqiError(RuntimeError):
@@ -210,6 +211,14 @@ class Context:
         res = RunResult()
         self._chk(self._L.mqi_run(self.h, ctypes.byref(cfg), ctypes.byref(res)), "mqi_run")
         return res
+
+    def autotune(self, cfg, sample=0.1):
+        """Fill cfg's launch parameters with the fastest measured on a sample
+        of the plan; returns (cfg, device ms of the best trial)."""
+        ms = ctypes.c_double()
+        self._chk(self._L.mqi_autotune(self.h, ctypes.byref(cfg), ctypes.c_double(sample), ctypes.byref(ms)),
+                  "mqi_autotune")
+        return cfg, ms.value
 
     def run_vertices(self, vertices, cfg, spot_ids=None):
         v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 7)

@@ -23,6 +23,15 @@
 namespace
 {
 
+// CSR-Adaptive segmentation of one CSR matrix (see mqi_aux::csr_adaptive)
+struct csr_plan {
+    int64_t   n_seg = 0, n_split = 0;
+    int64_t*  seg_k     = nullptr;   // n_seg + 1
+    uint32_t* seg_info  = nullptr;   // n_seg + 1
+    int32_t*  split_seg = nullptr;   // 2 per split row
+    double*   part      = nullptr;   // n_seg
+};
+
 struct csr_dev {
     int64_t  rows = 0, cols = 0, nnz = 0;
     int64_t* indptr  = nullptr;   // rows + 1
@@ -31,6 +40,7 @@ struct csr_dev {
     int64_t* tindptr = nullptr;   // transposed (voxel-major) copy: cols + 1
     int32_t* tcol    = nullptr;
     float*   tval    = nullptr;
+    csr_plan plan, tplan;         // segmentations of the spot-major and voxel-major forms
 };
 
 }   // namespace
@@ -52,6 +62,7 @@ struct mqi_ctx {
     std::vector<mqi_spot> spots;
     mqi_spot*             d_spots = nullptr;
     int                   n_scen  = 1;
+    bool                  scen_set = false;   // mqi_set_scenarios gave explicit shifts / scales
     float *               d_shift = nullptr, *d_scale = nullptr;
 
     // scorers
@@ -86,7 +97,8 @@ struct mqi_ctx {
     std::vector<csr_dev> dij;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double     last_matvec_ms = 0.0;
+    hipEvent_t ev_mv0 = nullptr, ev_mv1 = nullptr;   // around the last sparse product
+    bool       matvec_pending = false;
 };
 
 namespace
@@ -147,8 +159,59 @@ free_dij(mqi_ctx* c) {
         dfree(m.tindptr);
         dfree(m.tcol);
         dfree(m.tval);
+        for (csr_plan* p : {&m.plan, &m.tplan}) {
+            dfree(p->seg_k);
+            dfree(p->seg_info);
+            dfree(p->split_seg);
+            dfree(p->part);
+        }
     }
     c->dij.clear();
+}
+
+// CSR-Adaptive segmentation of a row pointer array: rows longer than
+// MQI_SEG_SHORT are cut into MQI_SEG_LONG pieces, shorter rows are packed
+// (MQI_SEG_SHORT entries, MQI_SEG_ROWS rows at most per segment).
+int
+build_plan(mqi_ctx* c, const std::vector<int64_t>& ip, csr_plan& P) {
+    const int64_t         rows = (int64_t) ip.size() - 1;
+    std::vector<int64_t>  sk;
+    std::vector<uint32_t> si;
+    std::vector<int32_t>  split;
+    if (rows >= (1ll << 31)) return fail(c, MQI_E_ARG, "more than 2^31 - 1 rows");
+    int64_t r = 0;
+    while (r < rows) {
+        const int64_t len = ip[r + 1] - ip[r];
+        if (len > MQI_SEG_SHORT) {
+            split.push_back((int32_t) sk.size());
+            for (int64_t off = 0; off < len; off += MQI_SEG_LONG) {
+                sk.push_back(ip[r] + off);
+                si.push_back((uint32_t) r | 0x80000000u);
+            }
+            split.push_back((int32_t) sk.size());
+            ++r;
+        } else {
+            const int64_t r0 = r, kb = ip[r];
+            while (r < rows && r - r0 < MQI_SEG_ROWS && ip[r + 1] - kb <= MQI_SEG_SHORT) ++r;
+            sk.push_back(kb);
+            si.push_back((uint32_t) r0);
+        }
+        if (sk.size() >= (size_t) INT32_MAX) return fail(c, MQI_E_ARG, "too many CSR segments");
+    }
+    P.n_seg   = (int64_t) sk.size();
+    P.n_split = (int64_t) split.size() / 2;
+    sk.push_back(ip[rows]);
+    si.push_back((uint32_t) rows);
+    int e;
+    if ((e = dalloc(c, &P.seg_k, sk.size())) || (e = dalloc(c, &P.seg_info, si.size())) ||
+        (e = dalloc(c, &P.split_seg, std::max<size_t>(split.size(), 1))) ||
+        (e = dalloc(c, &P.part, std::max<size_t>(P.n_seg, 1))))
+        return e;
+    HIPCHK(c, hipMemcpy(P.seg_k, sk.data(), sizeof(int64_t) * sk.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(P.seg_info, si.data(), sizeof(uint32_t) * si.size(), hipMemcpyHostToDevice));
+    if (!split.empty())
+        HIPCHK(c, hipMemcpy(P.split_seg, split.data(), sizeof(int32_t) * split.size(), hipMemcpyHostToDevice));
+    return 0;
 }
 
 uint64_t
@@ -261,8 +324,8 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     std::memcpy(P.fwd, c->fwd, sizeof P.fwd);
     std::memcpy(P.inv, c->inv, sizeof P.inv);
     std::memcpy(P.trans, c->trans, sizeof P.trans);
-    P.scen_shift    = c->n_scen > 1 ? c->d_shift : nullptr;
-    P.scen_scale    = c->n_scen > 1 ? c->d_scale : nullptr;
+    P.scen_shift    = c->scen_set ? c->d_shift : nullptr;
+    P.scen_scale    = c->scen_set ? c->d_scale : nullptr;
     P.spots         = c->d_spots;
     P.n_spots       = (int) c->spots.size();
     P.item_end      = c->d_item_end;
@@ -338,6 +401,7 @@ check_tuning(mqi_ctx* c, const mqi_run_config& cfg) {
     if (cfg.block_size != 0 && (cfg.block_size % 64 != 0 || cfg.block_size > 256 || cfg.block_size < 0))
         return fail(c, MQI_E_ARG, "block_size must be 64, 128, 192 or 256");
     if (cfg.blocks < 0) return fail(c, MQI_E_ARG, "blocks must be >= 0");
+    if (cfg.stack_depth < 0 || cfg.stack_depth > 4096) return fail(c, MQI_E_ARG, "stack_depth must be in [0, 4096]");
     return 0;
 }
 
@@ -494,6 +558,8 @@ mqi_create(int device, mqi_ctx** out) {
     if (c->n_cu <= 0) c->n_cu = 1;
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
+    hipEventCreate(&c->ev_mv0);
+    hipEventCreate(&c->ev_mv1);
     int r;
     if ((r = dalloc(c, &c->d_counters, 4)) || (r = dalloc(c, &c->d_queue, MQI_QUEUES * MQI_QUEUE_STRIDE)) || (r = dalloc(c, &c->d_status, 1)) || (r = dalloc(c, &c->d_hoff, 1)) ||
         (r = dalloc(c, &c->d_statmax, 1)) || (r = dalloc(c, &c->d_statsum, 1)) || (r = dalloc(c, &c->d_statcnt, 1))) {
@@ -540,6 +606,8 @@ mqi_destroy(mqi_ctx* c) {
     free_dij(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->ev_mv0) hipEventDestroy(c->ev_mv0);
+    if (c->ev_mv1) hipEventDestroy(c->ev_mv1);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -618,14 +686,16 @@ mqi_set_scenarios(mqi_ctx* c, int n, const float* shift, const float* scale) {
     if (!c || n < 0) return MQI_E_ARG;
     hipSetDevice(c->device);
     if (n == 0) {
-        c->n_scen = 1;
+        c->n_scen   = 1;
+        c->scen_set = false;
     } else {
         if (!shift || !scale) return fail(c, MQI_E_ARG, "scenario arrays are NULL");
         int r;
         if ((r = dalloc(c, &c->d_shift, 3 * (size_t) n)) || (r = dalloc(c, &c->d_scale, (size_t) n))) return r;
         HIPCHK(c, hipMemcpy(c->d_shift, shift, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_scale, scale, sizeof(float) * n, hipMemcpyHostToDevice));
-        c->n_scen = n;
+        c->n_scen   = n;
+        c->scen_set = true;
     }
     if (c->mask && c->per_scen) return mqi_set_scorers(c, c->mask, c->per_scen);
     return 0;
@@ -730,6 +800,85 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
         return fail(c, MQI_E_CAPACITY, "a Dij table overflowed: raise dij_capacity above %llu",
                     (unsigned long long) c->dij_cap);
     return 0;
+}
+
+// One timed batched pass of the current plan (counters cleared first);
+// returns the device time and the number of dropped secondaries.
+int
+trial_pass(mqi_ctx* c, const mqi_run_config& cfg, double* ms, unsigned long long* dropped) {
+    KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 4, c->stream));
+    double t = 0;
+    int    r = transport_pass(c, cfg, 0, &t);
+    if (r) return r;
+    unsigned long long cnt[4];
+    HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
+    *ms      = t;
+    *dropped = cnt[2];
+    return 0;
+}
+
+int
+mqi_autotune(mqi_ctx* c, mqi_run_config* cfg, double sample, double* best_ms) {
+    int r = ready(c);
+    if (r) return r;
+    if (!cfg || !(sample > 0.0) || sample > 1.0) return fail(c, MQI_E_ARG, "bad autotune arguments");
+    if (c->spots.empty()) return fail(c, MQI_E_STATE, "no spots: call mqi_set_spots first");
+    if (c->kinds.empty()) return fail(c, MQI_E_STATE, "no scorers");
+    if (cfg->exact) return fail(c, MQI_E_ARG, "the verification kernels are not tuned");
+    hipSetDevice(c->device);
+    mqi_run_config base = *cfg;
+    const bool     tune_stack = base.stack_depth < 0;
+    base.stack_depth          = tune_stack ? 0 : base.stack_depth;
+    base.batched              = 1;
+    base.dij                  = 0;
+    base.stat_criterion       = 0;
+    base.blocks               = 0;
+    if ((r = check_tuning(c, base))) return r;
+    // trial plan: every spot's histories scaled by `sample` (at least one)
+    struct restore_spots {
+        mqi_ctx*              c;
+        std::vector<mqi_spot> saved;
+        ~restore_spots() { c->spots = saved; }
+    } guard{c, c->spots};
+    for (auto& sp : c->spots)
+        if (sp.histories) sp.histories = std::max<uint64_t>(1, (uint64_t) std::llround((double) sp.histories * sample));
+    double             ms = 0;
+    unsigned long long dropped = 0;
+    if ((r = trial_pass(c, base, &ms, &dropped))) return r;   // warm-up
+    if (tune_stack) {
+        // the smallest secondary stack that drops nothing on the sample
+        const int depths[] = {16, 32, 64, 128, 256, 512};
+        for (int d : depths) {
+            base.stack_depth = d;
+            if ((r = trial_pass(c, base, &ms, &dropped))) return r;
+            if (dropped == 0) break;
+        }
+    }
+    const int waves[]  = {3, 4, 5};
+    const int blocks[] = {128, 256};
+    const int chunks[] = {1, 2, 4};
+    double    best     = 1e300;
+    mqi_run_config pick = base;
+    for (int w : waves)
+        for (int b : blocks)
+            for (int ch : chunks) {
+                mqi_run_config t = base;
+                t.waves_per_simd = w;
+                t.block_size     = b;
+                t.fetch_chunk    = ch;
+                if ((r = trial_pass(c, t, &ms, &dropped))) return r;
+                if (ms < best) {
+                    best = ms;
+                    pick = t;
+                }
+            }
+    cfg->waves_per_simd = pick.waves_per_simd;
+    cfg->block_size     = pick.block_size;
+    cfg->fetch_chunk    = pick.fetch_chunk;
+    if (tune_stack) cfg->stack_depth = pick.stack_depth;
+    if (best_ms) *best_ms = best;
+    // trial passes scored into the scorers
+    return mqi_zero_scorers(c);
 }
 
 int
@@ -915,6 +1064,7 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             (r = dalloc(c, &m.val, std::max<int64_t>(1, m.nnz))))
             return r;
         HIPCHK(c, hipMemcpy(m.indptr, indptr.data(), sizeof(int64_t) * (n_spots + 1), hipMemcpyHostToDevice));
+        if ((r = build_plan(c, indptr, m.plan))) return r;
         KCHK(c, mqi_aux::dij_write(c->d_dij_keys + (size_t) s * n_spots * c->dij_cap,
                                    c->d_dij_vals + (size_t) s * n_spots * c->dij_cap, c->dij_cap, n_spots, rel_thr,
                                    d_max + (size_t) s * n_spots, m.indptr, m.col, m.val, c->stream));
@@ -948,6 +1098,7 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             std::vector<int64_t> tptr(m.cols + 1, 0);
             for (int64_t v = 0; v < m.cols; ++v) tptr[v + 1] = tptr[v] + cc[v];
             HIPCHK(c, hipMemcpy(m.tindptr, tptr.data(), sizeof(int64_t) * (m.cols + 1), hipMemcpyHostToDevice));
+            if ((r = build_plan(c, tptr, m.tplan))) return r;
             HIPCHK(c, hipMemcpy(d_cur, tptr.data(), sizeof(int64_t) * m.cols, hipMemcpyHostToDevice));
             KCHK(c, mqi_aux::csr_transpose_scatter(m.indptr, m.col, m.val, m.rows, d_cur, m.tcol, m.tval, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -999,35 +1150,40 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
     hipSetDevice(c->device);
     const csr_dev& m  = c->dij[s];
     hipStream_t    st = stream ? (hipStream_t) stream : c->stream;
-    HIPCHK(c, hipEventRecord(c->ev0, st));
+    HIPCHK(c, hipEventRecord(c->ev_mv0, st));
     if (transpose) {
         // D^T g: rows of the spot-major matrix
-        KCHK(c, mqi_aux::csr_rows(m.indptr, m.col, m.val, x, y, y_double, m.rows, alpha, accumulate, 0, st));
+        const csr_plan& p = m.plan;
+        KCHK(c, mqi_aux::csr_adaptive(m.indptr, m.col, m.val, x, p.seg_k, p.seg_info, p.n_seg, p.split_seg, p.n_split,
+                                      p.part, y, y_double, alpha, accumulate, st));
+    } else if (m.nnz == 0) {
+        // D w of an empty matrix: zeros (or y unchanged)
+        if (!accumulate) {
+            if (y_double)
+                KCHK(c, mqi_aux::fill_u64(y, 0ull, m.cols, st));
+            else
+                KCHK(c, mqi_aux::fill_u32(y, 0u, m.cols, st));
+        }
     } else {
         // D w: rows of the voxel-major copy
-        if (m.nnz == 0) {
-            if (!accumulate) {
-                if (y_double)
-                    KCHK(c, mqi_aux::fill_u64(y, 0ull, m.cols, st));
-                else
-                    KCHK(c, mqi_aux::fill_u32(y, 0u, m.cols, st));
-            }
-        } else {
-            KCHK(c, mqi_aux::csr_rows(m.tindptr, m.tcol, m.tval, x, y, y_double, m.cols, alpha, accumulate, 0, st));
-        }
+        const csr_plan& p = m.tplan;
+        KCHK(c, mqi_aux::csr_adaptive(m.tindptr, m.tcol, m.tval, x, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
+                                      p.n_split, p.part, y, y_double, alpha, accumulate, st));
     }
-    HIPCHK(c, hipEventRecord(c->ev1, st));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float t = 0;
-    hipEventElapsedTime(&t, c->ev0, c->ev1);
-    c->last_matvec_ms = t;
+    HIPCHK(c, hipEventRecord(c->ev_mv1, st));
+    c->matvec_pending = true;
     return 0;
 }
 
 int
 mqi_last_matvec_ms(mqi_ctx* c, double* ms) {
     if (!c || !ms) return MQI_E_ARG;
-    *ms = c->last_matvec_ms;
+    if (!c->matvec_pending) return fail(c, MQI_E_STATE, "no sparse product was run");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipEventSynchronize(c->ev_mv1));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev_mv0, c->ev_mv1));
+    *ms = t;
     return 0;
 }
 

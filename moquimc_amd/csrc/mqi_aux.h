@@ -23,8 +23,14 @@ int dij_write(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n
 int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, int64_t nnz, int n_seg,
                    const int64_t* seg_begin, const int64_t* seg_end, void* scratch, size_t* scratch_bytes,
                    hipStream_t s);
-int csr_rows(const int64_t* indptr, const int32_t* col, const float* val, const float* x, void* y, int y_is_double,
-             int64_t n_rows, double alpha, int accumulate, int blocks, hipStream_t s);
+// CSR-Adaptive segment sizes (entries of one long-row piece; entries and rows
+// of one packed segment)
+#define MQI_SEG_LONG 16384
+#define MQI_SEG_SHORT 1024
+#define MQI_SEG_ROWS 1024
+int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
+                 const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+                 void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
 int csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t s);
 int csr_transpose_scatter(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
                           unsigned long long* cursor, int32_t* tcol, float* tval, hipStream_t s);

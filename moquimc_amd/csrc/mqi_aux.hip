@@ -123,7 +123,9 @@ __global__ void k_stat_max(const double* s1, uint64_t n, double N, float* out_ma
 
 __global__ void k_stat_sum(const double* s1, const double* s2, uint64_t n, double N, const float* max_mu, float thr,
                            double* out_sum, unsigned long long* out_cnt) {
-    const float lim = *max_mu * thr;
+    // dose_mean[i] > dose_max * stat_threshold with dose_max a double
+    // (mqi_tps_env.hpp:1409-1425)
+    const double lim = (double) *max_mu * (double) thr;
     double      acc = 0.0;
     unsigned long long cnt = 0;
     for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x) {
@@ -265,55 +267,132 @@ segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, i
 }
 
 // ------------------------------------------------------------------ sparse products
-// Row-wise CSR product y[r] = sum_k val[k] * x[col[k]] with FP64 accumulation.
-// One wavefront per row, rows interleaved over the grid; values and column
-// indices are streamed once, x is gathered (it stays in the last-level cache).
+// y = alpha * A x (+ y) for a CSR matrix A (float values, int32 columns) with
+// FP64 accumulation, in the CSR-Adaptive form (J. L. Greathouse and M. Daga,
+// "Efficient sparse matrix-vector multiplication on GPUs using the CSR storage
+// format", SC'14): the nonzeros are tiled into segments at build time.
+//   partial segment  a piece (<= MQI_SEG_LONG entries) of one long row; the
+//                    wave streams it with 16-byte loads and leaves its sum in
+//                    part[s]; a second pass adds the pieces of each split row
+//                    in order.
+//   packed segment   whole rows, <= MQI_SEG_SHORT entries and <= MQI_SEG_ROWS
+//                    rows: the wave forms the products into LDS with
+//                    coalesced loads, then each lane sums one row.
+// Every sum runs in a fixed order, so results do not depend on scheduling.
+// seg_info[s] = first row | (partial << 31); seg_k[s] = first nonzero;
+// entry n_seg of both is the sentinel (rows, nnz).
+typedef float   f32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
 template<typename TY>
-__global__ void __launch_bounds__(256) k_csr_rows(const int64_t* __restrict__ indptr, const int32_t* __restrict__ col,
-                                                  const float* __restrict__ val, const float* __restrict__ x,
-                                                  TY* __restrict__ y, int64_t n_rows, double alpha, int accumulate) {
-    const int     lane  = threadIdx.x & 63;
-    const int64_t wave  = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
-    for (int64_t r = wave; r < n_rows; r += waves) {
-        const int64_t b = indptr[r], e = indptr[r + 1];
-        double        acc = 0.0;
-        int64_t       k   = b + lane;
-        // 4 independent loads in flight per lane
-        for (; k + 192 < e; k += 256) {
-            const float   v0 = val[k], v1 = val[k + 64], v2 = val[k + 128], v3 = val[k + 192];
-            const int32_t c0 = col[k], c1 = col[k + 64], c2 = col[k + 128], c3 = col[k + 192];
-            acc += (double) v0 * (double) x[c0];
-            acc += (double) v1 * (double) x[c1];
-            acc += (double) v2 * (double) x[c2];
-            acc += (double) v3 * (double) x[c3];
-        }
-        for (; k < e; k += 64) acc += (double) val[k] * (double) x[col[k]];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) {
-            const double out = alpha * acc;
-            if (accumulate)
-                y[r] = (TY) ((double) y[r] + out);
-            else
-                y[r] = (TY) out;
+__global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const float* __restrict__ val, const float* __restrict__ x,
+                                                     const int64_t* __restrict__ seg_k,
+                                                     const uint32_t* __restrict__ seg_info, int64_t n_seg,
+                                                     double* __restrict__ part, TY* __restrict__ y, double alpha,
+                                                     int accumulate) {
+    __shared__ double buf[MQI_SEG_SHORT];
+    const int lane = threadIdx.x;
+    // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
+    // walks a contiguous run of segments (neighbouring segments gather the same
+    // lines of x, which then stay in that XCD's L2).  gridDim.x is a multiple
+    // of 8, so the remap is a permutation of [0, gridDim.x).
+    const int64_t per_xcd = gridDim.x >> 3;
+    const int64_t first   = (int64_t) (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    for (int64_t s = first; s < n_seg; s += gridDim.x) {
+        const uint32_t info = seg_info[s];
+        const int64_t  kb = seg_k[s], ke = seg_k[s + 1];
+        if (info >> 31) {
+            double  acc = 0.0;
+            int64_t ka  = (kb + 3) & ~(int64_t) 3;
+            if (ka > ke) ka = ke;
+            if (lane < ka - kb) acc += (double) val[kb + lane] * (double) x[col[kb + lane]];
+            const int64_t n4 = (ke - ka) >> 2;
+            const f32x4*  v4 = reinterpret_cast<const f32x4*>(val + ka);
+            const i32x4*  c4 = reinterpret_cast<const i32x4*>(col + ka);
+            int64_t       i  = lane;
+            for (; i + 192 < n4; i += 256) {
+                f32x4 v[4];
+                i32x4 c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    v[u] = __builtin_nontemporal_load(v4 + i + 64 * u);
+                    c[u] = __builtin_nontemporal_load(c4 + i + 64 * u);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc += (double) v[u].x * (double) x[c[u].x];
+                    acc += (double) v[u].y * (double) x[c[u].y];
+                    acc += (double) v[u].z * (double) x[c[u].z];
+                    acc += (double) v[u].w * (double) x[c[u].w];
+                }
+            }
+            for (; i < n4; i += 64) {
+                const f32x4 v = __builtin_nontemporal_load(v4 + i);
+                const i32x4 c = __builtin_nontemporal_load(c4 + i);
+                acc += (double) v.x * (double) x[c.x];
+                acc += (double) v.y * (double) x[c.y];
+                acc += (double) v.z * (double) x[c.z];
+                acc += (double) v.w * (double) x[c.w];
+            }
+            const int64_t kt = ka + (n4 << 2);
+            if (lane < ke - kt) acc += (double) val[kt + lane] * (double) x[col[kt + lane]];
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+            if (lane == 0) part[s] = acc;
+        } else {
+            const int64_t r0 = info, r1 = seg_info[s + 1] & 0x7fffffffu;
+            for (int64_t k = kb + lane; k < ke; k += 64) buf[k - kb] = (double) val[k] * (double) x[col[k]];
+            __syncthreads();
+            for (int64_t r = r0 + lane; r < r1; r += 64) {
+                double        acc = 0.0;
+                const int64_t e   = indptr[r + 1] - kb;
+                for (int64_t k = indptr[r] - kb; k < e; ++k) acc += buf[k];
+                const double out = alpha * acc;
+                y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+            }
+            __syncthreads();
         }
     }
 }
 
+// rows split over partial segments [split_seg[2i], split_seg[2i+1]):
+// y[r] = alpha * sum of the pieces in order (+ y[r])
+template<typename TY>
+__global__ void k_csr_split_rows(const uint32_t* __restrict__ seg_info, const int32_t* __restrict__ split_seg,
+                                 int64_t n_split, const double* __restrict__ part, TY* __restrict__ y, double alpha,
+                                 int accumulate) {
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_split) return;
+    const int32_t s0 = split_seg[2 * i], s1 = split_seg[2 * i + 1];
+    const int64_t r  = seg_info[s0] & 0x7fffffffu;
+    double        acc = 0.0;
+    for (int32_t s = s0; s < s1; ++s) acc += part[s];
+    const double out = alpha * acc;
+    y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+}
+
 int
-csr_rows(const int64_t* indptr, const int32_t* col, const float* val, const float* x, void* y, int y_is_double,
-         int64_t n_rows, double alpha, int accumulate, int blocks, hipStream_t s) {
-    if (n_rows <= 0) return 0;
-    if (blocks <= 0) {
-        int64_t b = (n_rows + 3) / 4;
-        blocks    = (int) (b > 16384 ? 16384 : b);
-    }
+csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
+             const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+             void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+    if (n_seg <= 0) return 0;
+    const int blocks = (int) (((n_seg < 262144 ? n_seg : 262144) + 7) & ~(int64_t) 7);
     if (y_is_double)
-        hipLaunchKernelGGL(k_csr_rows<double>, dim3(blocks), dim3(256), 0, s, indptr, col, val, x, (double*) y, n_rows,
-                           alpha, accumulate);
+        hipLaunchKernelGGL(k_csr_adaptive<double>, dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k, seg_info,
+                           n_seg, part, (double*) y, alpha, accumulate);
     else
-        hipLaunchKernelGGL(k_csr_rows<float>, dim3(blocks), dim3(256), 0, s, indptr, col, val, x, (float*) y, n_rows,
-                           alpha, accumulate);
+        hipLaunchKernelGGL(k_csr_adaptive<float>, dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k, seg_info,
+                           n_seg, part, (float*) y, alpha, accumulate);
+    if (n_split > 0) {
+        const int b = (int) ((n_split + 255) / 256);
+        if (y_is_double)
+            hipLaunchKernelGGL(k_csr_split_rows<double>, dim3(b), dim3(256), 0, s, seg_info, split_seg, n_split, part,
+                               (double*) y, alpha, accumulate);
+        else
+            hipLaunchKernelGGL(k_csr_split_rows<float>, dim3(b), dim3(256), 0, s, seg_info, split_seg, n_split, part,
+                               (float*) y, alpha, accumulate);
+    }
     return (int) hipGetLastError();
 }
 

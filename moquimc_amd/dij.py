@@ -1,0 +1,84 @@
+"""Differentiable dose from a dose-influence matrix held on the device.
+
+A Dij built by ``Context.dij_build`` is a CSR matrix with one row per spot
+(the orientation of the reference's ``io::save_to_npz``,
+moqui/base/mqi_io.hpp:249-320).  With D the voxels x spots influence
+matrix, the dose of spot weights w is D w and the gradient of an objective
+f(D w) with respect to w is D^T (df/dd).  Both products run in the
+library's CSR-Adaptive kernels (FP64 accumulation, float32 inputs) on
+torch's current stream; ``dose`` wraps them in a ``torch.autograd.Function``.
+
+    mat = DijMatrix(ctx, scenario=0)
+    w = torch.rand(mat.n_spots, device="cuda", requires_grad=True)
+    f = quadratic_objective(mat, w, prescription)
+    f.backward()                      # w.grad = 2 D^T (D w - p) / n
+
+Robust optimisation: ``worst_case_objective`` over the Dij of several
+scenarios (max over scenarios, subgradient of the worst one).
+"""
+import torch
+
+
+class DijMatrix:
+    """Handle on the device CSR of one scenario of a context's Dij."""
+
+    def __init__(self, ctx, scenario=0):
+        self.ctx = ctx
+        self.scenario = int(scenario)
+        self.nnz, self.n_spots, self.n_voxels = ctx.dij_info(self.scenario)
+
+    def _vec(self, x, n):
+        if x.dim() != 1 or x.numel() != n:
+            raise ValueError("expected a vector of %d entries, got shape %s" % (n, tuple(x.shape)))
+        if not x.is_cuda:
+            raise ValueError("the sparse products take device tensors")
+        return x.detach().to(torch.float32).contiguous()
+
+    def _product(self, transpose, x, n_out, out_dtype):
+        y = torch.empty(n_out, dtype=torch.float64, device=x.device)
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        self.ctx.dij_matvec(self.scenario, int(transpose), x.data_ptr(), y.data_ptr(), y_double=True,
+                            stream=stream)
+        return y if out_dtype == torch.float64 else y.to(out_dtype)
+
+    def matvec(self, w, out_dtype=torch.float64):
+        """Dose D w (n_voxels)."""
+        return self._product(False, self._vec(w, self.n_spots), self.n_voxels, out_dtype)
+
+    def rmatvec(self, g, out_dtype=torch.float64):
+        """D^T g (n_spots)."""
+        return self._product(True, self._vec(g, self.n_voxels), self.n_spots, out_dtype)
+
+
+class _Dose(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, w, mat):
+        fctx.mat = mat
+        fctx.w_dtype = w.dtype
+        return mat.matvec(w, torch.float64)
+
+    @staticmethod
+    def backward(fctx, grad_dose):
+        grad_w = fctx.mat.rmatvec(grad_dose, torch.float64)
+        return grad_w.to(fctx.w_dtype), None
+
+
+def dose(mat, w):
+    """Differentiable dose D w (float64, n_voxels) of spot weights w."""
+    return _Dose.apply(w, mat)
+
+
+def quadratic_objective(mat, w, prescription, voxel_weights=None):
+    """f(w) = mean_v c_v (D w - p)_v^2 (c = 1 when voxel_weights is None)."""
+    d = dose(mat, w) - prescription.to(torch.float64)
+    r = d * d
+    if voxel_weights is not None:
+        r = r * voxel_weights.to(torch.float64)
+    return r.mean()
+
+
+def worst_case_objective(mats, w, prescription, voxel_weights=None):
+    """max over scenarios of the quadratic objective, and the per-scenario
+    values (the gradient flows through the worst scenario)."""
+    vals = torch.stack([quadratic_objective(m, w, prescription, voxel_weights) for m in mats])
+    return vals.max(), vals

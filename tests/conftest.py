@@ -13,8 +13,15 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
-def ctx():
+def _session_ctx():
     from moquimc_amd import _abi
     c = _abi.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture
+def ctx(_session_ctx):
+    """The session's context with the nominal scenario only."""
+    _session_ctx.set_scenarios()
+    return _session_ctx

@@ -12,8 +12,8 @@ Fixtures (inputs and the reference's outputs, nothing else):
   reference_trace.npz       per-step vacuum ray traversal of the transport kernel
   reference_transport.npz   dose / LETd / stat scorers of short seeded runs
                             (vertices and seeds included), debug and release
-  reference_profiles.npz    (--stats) depth-dose and lateral profiles of
-                            high-statistics runs for the statistical GPU tests
+  reference_profiles.npz    (--stats) high-statistics dose of the water and
+                            slab workloads for the statistical GPU tests
 """
 import argparse
 import os
@@ -24,6 +24,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 import phantoms  # noqa: E402
 import refbind as rb  # noqa: E402
 
@@ -138,39 +139,84 @@ def transport_fixture():
     return out
 
 
+def _batches(g, spots, dbg, per_spot, n_batches, mask, threads, seed0):
+    """n_batches independent reference runs of the spot list (per_spot
+    histories per spot each); returns the per-batch scorer arrays divided by
+    the histories of the batch."""
+    out = []
+    for b in range(n_batches):
+        vtx, seeds = [], []
+        for k, sp in enumerate(spots):
+            v, sd = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], seed0 + b, k, per_spot, debug=dbg)
+            vtx.append(v)
+            seeds.append(sd)
+        vtx = np.concatenate(vtx)
+        seeds = np.concatenate(seeds)
+        arrs, tracked = rb.ref_transport(dbg, g, vtx, seeds, mask=mask, kernel=0, threads=threads)
+        out.append([a / tracked for a in arrs])
+        print("  batch %d: %d histories" % (b, tracked), flush=True)
+    return out
+
+
+def _mean_se(xs):
+    a = np.array(xs)
+    return a.mean(0), a.std(0, ddof=1) / np.sqrt(len(xs))
+
+
 def profiles_fixture(threads):
-    """High-statistics reference runs: water (C1 geometry, debug physics) and
-    one slab spot (release), summarised as depth-dose / lateral profiles with
-    batch standard errors (8 independent seeds)."""
+    """High-statistics reference runs for the statistical GPU tests.
+
+    c1  water 200x200x350 @ 0.5x0.5x1 mm, one 200 MeV spot (sigma 3 mm, no
+        energy spread), debug physics, 8 batches x 1.25e6 histories: dose in
+        the box where the mean exceeds 1% of its maximum, its standard error,
+        the integral depth dose and the integral dose.
+    c2  the slab phantom with the 25-spot plan (70-230 MeV, 1% spread,
+        sigma 4 mm), release physics: dose from 8 batches x 1.25e5 histories
+        per spot, LETd (numerator and denominator scored without the dose
+        scorer) from 4 batches x 2.5e4 per spot; central xz / yz planes and
+        the integral depth dose.
+    Dose is per history (Gy)."""
+    from moquimc_amd import workloads
     out = {}
-    cases = [
-        ("water", phantoms.water_box(100, 100, 350, dx=0.5, dz=1.0), rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5)),
-         1, 250000),
-        ("slab", phantoms.slab_box(80, 80, 300, dx=1.0, dz=1.0),
-         rb.make_spot(150.0, 4.0, 0, pos=(0, 0, 0.5), energy_const=False, sigma_e=1.5), 0, 250000),
-    ]
-    for name, g, sp, dbg, per_batch in cases:
-        dd, lat = [], []
-        t0 = time.time()
-        for b in range(8):
-            vtx, seeds = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], 100 + b, 0, per_batch, debug=dbg)
-            (dose,), tracked = rb.ref_transport(dbg, g, vtx, seeds, mask=rb.SC_DOSE, kernel=0, threads=threads)
-            d3 = dose.reshape(g.nz, g.ny, g.nx)
-            dd.append(d3.sum(axis=(1, 2)) / per_batch)
-            iz = [g.nz - 1 - k for k in (20, 100, 150)]
-            lat.append(np.stack([d3[z, g.ny // 2, :] + d3[z, g.ny // 2 - 1, :] for z in iz]) / per_batch)
-        dd = np.array(dd)
-        lat = np.array(lat)
-        out[name + "_xe"], out[name + "_ye"], out[name + "_ze"] = g.xe, g.ye, g.ze
-        out[name + "_density"] = g.density[:: g.nx * g.ny]
-        out[name + "_spot"] = np.array([sp], rb.SPOT_DTYPE)
-        out[name + "_debug"] = np.array(dbg)
-        out[name + "_histories"] = np.array(8 * per_batch)
-        out[name + "_depth_dose"] = dd.mean(0)
-        out[name + "_depth_dose_se"] = dd.std(0, ddof=1) / np.sqrt(8)
-        out[name + "_lateral"] = lat.mean(0)
-        out[name + "_lateral_se"] = lat.std(0, ddof=1) / np.sqrt(8)
-        print(name, "done in %.0f s" % (time.time() - t0), flush=True)
+    t0 = time.time()
+    g = phantoms.water_box(200, 200, 350, dx=0.5, dz=1.0)
+    sp = rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5))
+    runs = _batches(g, [sp], 1, 1250000, 8, rb.SC_DOSE, threads, 1000)
+    m, se = _mean_se([r[0] for r in runs])
+    d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
+    hot = np.argwhere(d3 > 0.01 * d3.max())
+    lo, hi = hot.min(0), hot.max(0) + 1
+    out["c1_box"] = np.concatenate([lo, hi]).astype(np.int32)   # z0 y0 x0 z1 y1 x1
+    out["c1_dose"] = d3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].astype(np.float32)
+    out["c1_dose_se"] = s3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].astype(np.float16)
+    idd = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+    out["c1_idd"], out["c1_idd_se"] = _mean_se(idd)
+    tot = np.array([r[0].sum() for r in runs])
+    out["c1_total"], out["c1_total_se"] = _mean_se(tot)
+    out["c1_histories"] = np.array(8 * 1250000)
+    print("c1 done in %.0f s" % (time.time() - t0), flush=True)
+
+    t0 = time.time()
+    w = workloads.slabs()
+    g = rb.Grid(w.xe, w.ye, w.ze, w.density)
+    spots = [s.copy() for s in w.spots]
+    runs = _batches(g, spots, 0, 125000, 8, rb.SC_DOSE, threads, 2000)
+    m, se = _mean_se([r[0] for r in runs])
+    d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
+    out["c2_xz"], out["c2_xz_se"] = d3[:, 100, :].astype(np.float32), s3[:, 100, :].astype(np.float32)
+    out["c2_yz"], out["c2_yz_se"] = d3[:, :, 100].astype(np.float32), s3[:, :, 100].astype(np.float32)
+    idd = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+    out["c2_idd"], out["c2_idd_se"] = _mean_se(idd)
+    tot = np.array([r[0].sum() for r in runs])
+    out["c2_total"], out["c2_total_se"] = _mean_se(tot)
+    out["c2_histories"] = np.array(8 * 125000 * len(spots))
+    runs = _batches(g, spots, 0, 25000, 4, rb.SC_LETD, threads, 3000)
+    num = np.sum([r[0] for r in runs], 0).reshape(350, 200, 200)
+    den = np.sum([r[1] for r in runs], 0).reshape(350, 200, 200)
+    out["c2_let_num_xz"], out["c2_let_den_xz"] = num[:, 100, :], den[:, 100, :]
+    out["c2_let_num_yz"], out["c2_let_den_yz"] = num[:, :, 100], den[:, :, 100]
+    out["c2_let_histories"] = np.array(4 * 25000 * len(spots))
+    print("c2 done in %.0f s" % (time.time() - t0), flush=True)
     return out
 
 
@@ -181,6 +227,9 @@ def main():
     args = ap.parse_args()
     if not rb.have_reference():
         raise SystemExit("build the reference harness first: make -C oracle")
+    if args.stats:
+        np.savez_compressed(os.path.join(HERE, "reference_profiles.npz"), **profiles_fixture(args.threads))
+        return
     tables = [np.zeros(600, np.float32) for _ in range(6)] + [np.zeros(3996, np.float32)]
     rb.reference(0).ref_tables(*[rb.ptr(t, rb.c_f) for t in tables])
     names = ["cs_p_ion", "dedx_restricted", "range_steps", "cs_pp_e", "cs_po_e", "cs_po_i", "density_correction"]
@@ -189,8 +238,6 @@ def main():
     np.savez_compressed(os.path.join(HERE, "reference_grid.npz"), **grid_probes())
     np.savez_compressed(os.path.join(HERE, "reference_trace.npz"), **trace_fixture())
     np.savez_compressed(os.path.join(HERE, "reference_transport.npz"), **transport_fixture())
-    if args.stats:
-        np.savez_compressed(os.path.join(HERE, "reference_profiles.npz"), **profiles_fixture(args.threads))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -104,3 +104,41 @@ def test_dij_products_fp64(ctx):
     want_grad = D @ gv.astype(np.float64)
     assert np.allclose(dose.cpu().numpy(), want_dose, rtol=1e-6, atol=1e-6 * np.abs(want_dose).max())
     assert np.allclose(grad.cpu().numpy(), want_grad, rtol=1e-6, atol=1e-6 * np.abs(want_grad).max())
+
+
+def test_dij_products_long_rows_and_accumulate(ctx):
+    """Rows long enough to be cut into several pieces (> 16384 entries), an
+    empty row, float32 output and accumulate / alpha against scipy in FP64."""
+    import torch
+    g = phantoms.slab_box(60, 60, 150)
+    sp = phantoms.spots([150.0, 120.0, 130.0], 20000, sigma=5.0, sigma_e_frac=0.01,
+                        positions=[(0, 0, 0.5), (8, 5, 0.5), (-6, -4, 0.5)])
+    sp[1]["histories"] = 0
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_scenarios()
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE)
+    res = ctx.run(ctx.config(seed=8, batched=True, dij=True, dij_capacity=1 << 18))
+    assert res.status == 0
+    ctx.dij_build(0.0)
+    D = _csr(ctx)
+    assert np.diff(D.indptr).max() > 3 * 16384
+    rng = np.random.default_rng(2)
+    w = rng.uniform(0.5, 1.5, D.shape[0]).astype(np.float32)
+    gv = rng.normal(size=D.shape[1]).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    tw, tg = torch.from_numpy(w).to(dev), torch.from_numpy(gv).to(dev)
+    want_dose = D.T @ w.astype(np.float64)
+    want_grad = D @ gv.astype(np.float64)
+    y0 = rng.normal(size=D.shape[0])
+    grad = torch.from_numpy(y0.copy()).to(dev)
+    ctx.dij_matvec(0, 1, tg.data_ptr(), grad.data_ptr(), y_double=True, alpha=-2.0, accumulate=True)
+    dose32 = torch.zeros(D.shape[1], dtype=torch.float32, device=dev)
+    ctx.dij_matvec(0, 0, tw.data_ptr(), dose32.data_ptr(), y_double=False)
+    torch.cuda.synchronize()
+    assert ctx.last_matvec_ms() > 0
+    got = grad.cpu().numpy()
+    assert np.allclose(got, y0 - 2.0 * want_grad, rtol=1e-6, atol=1e-9 * np.abs(want_grad).max())
+    assert np.allclose(dose32.cpu().numpy(), want_dose, rtol=2e-6, atol=1e-6 * np.abs(want_dose).max())
+    # the empty spot row
+    assert D[1].nnz == 0
