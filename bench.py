@@ -28,15 +28,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def transport_roofline(res, ms_per_pass, n_scorers):
-    # algorithmic bytes of the transport: per voxel step one density load
-    # (4 B) and one FP64 read-modify-write per scorer (8 B each)
-    bytes_per_pass = res.steps * (4.0 + 8.0 * n_scorers)
-    achieved = bytes_per_pass / (ms_per_pass * 1e-3) / 1e9
+def transport_roofline(steps, kernel_ms, launches, n_scorers):
+    """Algorithmic bytes of the transport: per voxel step one density load
+    (4 B) and one FP64 read-modify-write per scorer (8 B each), over the
+    average duration of one persistent launch (HIP events on its stream)."""
+    bytes_per_step = 4.0 + 8.0 * n_scorers
+    per_launch = steps / launches * bytes_per_step
+    ms_per_launch = kernel_ms / launches
+    achieved = per_launch / (ms_per_launch * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-            "kernel": "k_transport (batched)", "steps_per_pass": int(res.steps),
-            "bytes_per_step": 4 + 8 * n_scorers}
+            "kernel": "k_transport<release, dynamic> (persistent)", "launch_ms": round(ms_per_launch, 3),
+            "voxel_steps_per_launch": int(steps / launches), "bytes_per_step": bytes_per_step,
+            "atomics_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1)}
 
 
 def cpu_baseline(w, cores, target_s=20.0):
@@ -68,14 +72,16 @@ def cpu_baseline(w, cores, target_s=20.0):
     seeds = np.concatenate(seeds)
     pilot = min(len(seeds), 4000 * cores)
     t = []
-    rb.ref_transport(0, grid, vtx[:pilot], seeds[:pilot], mask=rb.SC_DOSE, kernel=0, threads=cores, timing=t)
+    mask = rb.SC_DOSE | rb.SC_STAT   # transport_particles_patient_stat with the bench's scorers
+    rb.ref_transport(0, grid, vtx[:pilot], seeds[:pilot], mask=mask, kernel=1, threads=cores, timing=t)
     rate = pilot / max(t[0], 1e-6)
     n = int(min(len(seeds), max(pilot, rate * target_s)))
     t = []
-    _, tracked = rb.ref_transport(0, grid, vtx[:n], seeds[:n], mask=rb.SC_DOSE, kernel=0, threads=cores, timing=t)
+    _, tracked = rb.ref_transport(0, grid, vtx[:n], seeds[:n], mask=mask, kernel=1, threads=cores, timing=t)
     return {"value": round(tracked / t[0], 1), "unit": "histories/s", "cores": cores, "kind": "reference",
             "sample": "%d histories of %d spots (every %dth spot of the plan, full history counts), reference "
-                      "transport_particles_patient on the CPU, release physics, %d threads, %.1f s"
+                      "transport_particles_patient_stat on the CPU (dose + stat scorers), release physics, "
+                      "%d threads, %.1f s"
                       % (tracked, len(picks), stride, cores, t[0])}
 
 
@@ -151,6 +157,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--exact", action="store_true", help="time the verification kernels instead")
+    ap.add_argument("--no-tune", action="store_true", help="default launch parameters instead of the autotuner's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the reference kernel of the HBM counter runs)")
     args = ap.parse_args()
@@ -166,36 +173,51 @@ def main():
     from moquimc_amd import _abi, workloads
     t0 = time.time()
     w = workloads.ct()
+    w.mask = _abi.SCORE_DOSE | _abi.SCORE_STAT
     log("workload generated in %.1f s: %d spots, %d histories" % (time.time() - t0, len(w.spots), w.histories))
     ctx = _abi.Context(0)
     w.load(ctx)
-    cfg = ctx.config(seed=11, batched=True, exact=args.exact)
+    n_scorers = ctx.scorer_count()
+    # statistical stopping: mean sd/mean <= 1 % over voxels above 50 % of the maximum
+    stat = dict(stat_criterion=0.01, stat_threshold=0.5, max_rounds=16)
+    cfg = ctx.config(seed=11, batched=True, exact=args.exact, **stat)
+    tuned = None
+    if not args.exact and not args.no_tune:
+        t0 = time.time()
+        cfg, best = ctx.autotune(cfg, sample=0.1)
+        tuned = {"waves_per_simd": cfg.waves_per_simd, "block_size": cfg.block_size, "fetch_chunk": cfg.fetch_chunk,
+                 "autotune_s": round(time.time() - t0, 2)}
+        log("autotuned: %s" % tuned)
     for i in range(args.warmup):
         ctx.zero_scorers()
         r = ctx.run(cfg)
-        log("warmup %d: %.3f s kernel, %d histories" % (i, r.kernel_ms / 1e3, r.histories))
-    kernel_ms, hist, last = 0.0, 0, None
+        log("warmup %d: %.3f s kernel, %d histories, %d rounds, stat %.4f" % (i, r.kernel_ms / 1e3, r.histories,
+                                                                             r.rounds, r.stat))
+    kernel_ms, hist, launches, steps, last = 0.0, 0, 0, 0, None
     tw0 = time.perf_counter()
     for i in range(args.steps):
         ctx.zero_scorers()
-        cfg.hist_offset = (i + 1) * 200000
+        cfg.hist_offset = (i + 1) * (1 << 24)
         last = ctx.run(cfg)
         kernel_ms += last.kernel_ms
         hist += last.histories
-        log("step %d: %.3f s kernel, %d histories, %d steps" % (i, last.kernel_ms / 1e3, last.histories, last.steps))
+        launches += last.rounds
+        steps += last.steps
+        log("step %d: %.3f s kernel, %d histories, %d rounds, stat %.4f, %d voxel steps" % (
+            i, last.kernel_ms / 1e3, last.histories, last.rounds, last.stat, last.steps))
     wall = time.perf_counter() - tw0
     ms_per_step = wall / args.steps * 1e3
     value = hist / wall
 
-    # plain per-spot launches
+    # plain per-spot launches (one launch and host sync per spot, as run_by_spot), same criterion
     plain_ms, plain_hist = 0.0, 0
     tp0 = time.perf_counter()
     for i in range(args.plain_passes):
         ctx.zero_scorers()
-        r = ctx.run(ctx.config(seed=11, batched=False, exact=args.exact, hist_offset=(i + 1) * 200000))
+        r = ctx.run(ctx.config(seed=11, batched=False, exact=args.exact, hist_offset=(i + 1) * (1 << 24), **stat))
         plain_ms += r.kernel_ms
         plain_hist += r.histories
-        log("plain pass %d: %.3f s kernel" % (i, r.kernel_ms / 1e3))
+        log("plain pass %d: %.3f s kernel, %d rounds" % (i, r.kernel_ms / 1e3, r.rounds))
     plain_wall = time.perf_counter() - tp0
     plain_rate = plain_hist / plain_wall if plain_hist else None
 
@@ -203,15 +225,17 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True,
            "scaling": "strong", "vs_baseline": None, "dtype": "f32 (FP64 dose accumulation)",
            "data": "synthetic (seeded CT and plan generators in moquimc_amd/workloads.py)",
-           "config": {"workload": w.name + ", one persistent launch per pass, release physics",
-                      "histories_per_step": int(w.histories), "spots": int(len(w.spots)),
-                      "grid": "512x512x200", "scorers": "dose"},
+           "config": {"workload": w.name + ", one persistent launch per round, stat stop 1% above 50% max, "
+                                           "release physics",
+                      "histories_per_round": int(w.histories), "spots": int(len(w.spots)),
+                      "grid": "512x512x200", "scorers": "dose, dose_stat, dose_square_stat", "tuned": tuned},
+           "rounds_per_step": round(launches / args.steps, 2),
            "kernel_histories_per_s": round(hist / (kernel_ms / 1e3), 1),
            "plain_histories_per_s": round(plain_rate, 1) if plain_rate else None,
            "plain_kernel_histories_per_s": round(plain_hist / (plain_ms / 1e3), 1) if plain_ms else None,
            "batched_over_plain": round(value / plain_rate, 3) if plain_rate else None,
-           "steps_per_history": round(last.steps / max(1, last.histories), 1),
-           "roofline": transport_roofline(last, kernel_ms / args.steps, 1)}
+           "steps_per_history": round(steps / max(1, hist), 1),
+           "roofline": transport_roofline(steps, kernel_ms, launches, n_scorers)}
     ctx.close()
     if not args.no_adjoint:
         try:

@@ -21,7 +21,7 @@ typedef struct mqi_spot {
     float    energy;        /* mean kinetic energy, MeV */
     float    sigma_energy;  /* Gaussian energy spread, MeV (ignored when energy_const) */
     int32_t  energy_const;  /* 1: const_1d energy (no random draw), 0: norm_1d */
-    uint32_t reserved;
+    uint32_t phsp;          /* MQI_PHSP_GAUSS (phsp_6d) or MQI_PHSP_UNIFORM (phsp_6d_uniform) */
     float    mean[6];       /* x, y, z (mm), x', y', z' in the beam frame */
     float    sigma[6];      /* sigmas of the same; sigma[5] is ignored */
     float    corr[2];       /* x-x' and y-y' correlation coefficients */
@@ -29,6 +29,15 @@ typedef struct mqi_spot {
     float    translation[3];/* mm */
     uint64_t histories;     /* histories of this spot */
 } mqi_spot;
+
+/* Phase-space distribution of a spot: GAUSS draws N(0,1) per variable
+ * (phsp_6d, mqi_phsp6d.hpp:62-79); UNIFORM draws U(-1,1) (phsp_6d_uniform,
+ * mqi_phsp6d_uniform.hpp:68-82, the distribution phantom_env uses,
+ * mqi_phantom_env.hpp:216); both combine the draws as described above. */
+enum {
+    MQI_PHSP_GAUSS   = 0,
+    MQI_PHSP_UNIFORM = 1
+};
 
 /* Scorer bits.  DOSE: dose-to-water (Gy per history sum,
  * mqi_scorer_energy_deposit.hpp:43-60); LETD: LETd numerator and

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libmqi_hip.so")
 
 SPOT_DTYPE = np.dtype([
     ("energy", np.float32), ("sigma_energy", np.float32), ("energy_const", np.int32),
-    ("reserved", np.uint32), ("mean", np.float32, 6), ("sigma", np.float32, 6),
+    ("phsp", np.uint32), ("mean", np.float32, 6), ("sigma", np.float32, 6),
     ("corr", np.float32, 2), ("rotation", np.float32, 9), ("translation", np.float32, 3),
     ("histories", np.uint64)])
 assert SPOT_DTYPE.itemsize == 128

@@ -68,6 +68,22 @@ def build(verbose=True):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def build_host(verbose=True):
+    """The C++ phantom driver over the library (moquimc_amd/host)."""
+    host = os.path.join(os.path.dirname(LIB), "host")
+    exe = os.path.join(host, "phantom_env")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + host,
+           os.path.join(host, "phantom_main.cpp"), "-L" + os.path.dirname(LIB), "-lmqi_hip",
+           "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 if __name__ == "__main__":
     build()
+    build_host()
     sys.exit(0)

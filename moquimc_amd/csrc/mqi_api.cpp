@@ -532,23 +532,35 @@ alloc_dij(mqi_ctx* c, const mqi_run_config& cfg) {
     return 0;
 }
 
+// message of the last failed mqi_create on this thread (there is no context
+// to hold it)
+thread_local std::string g_create_error = "no error";
+
+int
+create_failed(int code, const std::string& msg) {
+    g_create_error = msg;
+    return code;
+}
+
 }   // namespace
 
 extern "C" {
 
 int
 mqi_create(int device, mqi_ctx** out) {
-    if (!out) return MQI_E_ARG;
+    if (!out) return create_failed(MQI_E_ARG, "out is NULL");
     *out   = nullptr;
-    int nd = 0;
-    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return MQI_E_HIP;
-    if (device < 0 || device >= nd) return MQI_E_ARG;
+    int        nd = 0;
+    hipError_t e  = hipGetDeviceCount(&nd);
+    if (e != hipSuccess || nd <= 0)
+        return create_failed(MQI_E_HIP, std::string("no HIP device: ") + (e != hipSuccess ? hipGetErrorString(e) : "0 devices"));
+    if (device < 0 || device >= nd) return create_failed(MQI_E_ARG, "device index out of range");
     mqi_ctx* c = new (std::nothrow) mqi_ctx();
-    if (!c) return MQI_E_MEMORY;
+    if (!c) return create_failed(MQI_E_MEMORY, "out of host memory");
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&c->stream)) != hipSuccess) {
         delete c;
-        return MQI_E_HIP;
+        return create_failed(MQI_E_HIP, std::string("device set-up failed: ") + hipGetErrorString(e));
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -615,7 +627,7 @@ mqi_destroy(mqi_ctx* c) {
 
 const char*
 mqi_last_error(const mqi_ctx* c) {
-    return c ? c->err.c_str() : "no context";
+    return c ? c->err.c_str() : g_create_error.c_str();
 }
 
 int

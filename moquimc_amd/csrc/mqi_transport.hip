@@ -539,11 +539,20 @@ MQI_DI void sample_vertex(const transport_params& P, const mqi_spot& sp, uint32_
                           uint64_t h, trk& t) {
     rng_t r;
     r.s.init(P.seed, (uint32_t) h, spot_id, scenario, mqi_rng::STREAM_VERTEX);
-    float Ux = r.gauss();
-    float Vx = r.gauss();
-    float Uy = r.gauss();
-    float Vy = r.gauss();
-    float Uz = r.gauss();
+    float Ux, Vx, Uy, Vy, Uz;
+    if (sp.phsp == MQI_PHSP_UNIFORM) {   // U(-1, 1) draws (phsp_6d_uniform)
+        Ux = 2.0f * r.uniform() - 1.0f;
+        Vx = 2.0f * r.uniform() - 1.0f;
+        Uy = 2.0f * r.uniform() - 1.0f;
+        Vy = 2.0f * r.uniform() - 1.0f;
+        Uz = 2.0f * r.uniform() - 1.0f;
+    } else {
+        Ux = r.gauss();
+        Vx = r.gauss();
+        Uy = r.gauss();
+        Vy = r.gauss();
+        Uz = r.gauss();
+    }
     float ke = sp.energy;
     if (!sp.energy_const) {
         float z = r.gauss();

@@ -1178,8 +1178,9 @@ struct Transport {
 };
 
 // ---------------------------------------------------------------- sampling
-// beamlet::operator() with phsp_6d + norm_1d/const_1d (mqi_beamlet.hpp:81-90,
-// mqi_phsp6d.hpp:62-79, mqi_norm_1d.hpp:54-56): R = float.
+// beamlet::operator() with phsp_6d or phsp_6d_uniform + norm_1d/const_1d
+// (mqi_beamlet.hpp:81-90, mqi_phsp6d.hpp:62-79, mqi_phsp6d_uniform.hpp:68-82,
+// mqi_norm_1d.hpp:54-56): R = float.
 template<typename R>
 Vtx<R>
 phsp_vertex(const mqi_spot& sp, R Ux, R Vx, R Uy, R Vy, R Uz, R ke) {
@@ -1208,17 +1209,20 @@ phsp_vertex(const mqi_spot& sp, R Ux, R Vx, R Uy, R Vy, R Uz, R ke) {
 void
 sample_spot_std(uint64_t master_seed, uint32_t spot_index, const mqi_spot& sp, int64_t n,
                 Vtx<float>* out, int32_t* seeds) {
-    std::default_random_engine     beam_rng, tmp_rng;
-    std::normal_distribution<float> phsp_func(0, 1);
-    std::normal_distribution<float> e_func(sp.energy, sp.sigma_energy);
+    std::default_random_engine             beam_rng, tmp_rng;
+    std::normal_distribution<float>        phsp_func(0, 1);
+    std::uniform_real_distribution<float>  phsp_uni(-1.0, 1.0);   // phsp_6d_uniform (mqi_phsp6d_uniform.hpp:46)
+    std::normal_distribution<float>        e_func(sp.energy, sp.sigma_energy);
+    const bool                             uni = sp.phsp == MQI_PHSP_UNIFORM;
+    auto draw = [&]() { return uni ? phsp_uni(beam_rng) : phsp_func(beam_rng); };
     beam_rng.seed((uint32_t) (master_seed + spot_index));
     tmp_rng.seed((uint32_t) (master_seed + spot_index));
     for (int64_t h = 0; h < n; ++h) {
-        float Ux = phsp_func(beam_rng);
-        float Vx = phsp_func(beam_rng);
-        float Uy = phsp_func(beam_rng);
-        float Vy = phsp_func(beam_rng);
-        float Uz = phsp_func(beam_rng);
+        float Ux = draw();
+        float Vx = draw();
+        float Uy = draw();
+        float Vy = draw();
+        float Uz = draw();
         float ke = sp.energy_const ? sp.energy : e_func(beam_rng);
         out[h]   = phsp_vertex<float>(sp, Ux, Vx, Uy, Vy, Uz, ke);
         if (seeds) seeds[h] = (int32_t) tmp_rng();
@@ -1232,11 +1236,13 @@ sample_history_philox(uint64_t master_seed, uint32_t scenario, uint32_t spot_ind
                       const mqi_spot& sp) {
     mqi_rng::stream s;
     s.init(master_seed, h, spot_index, scenario, mqi_rng::STREAM_VERTEX);
-    R Ux = (R) s.gauss();
-    R Vx = (R) s.gauss();
-    R Uy = (R) s.gauss();
-    R Vy = (R) s.gauss();
-    R Uz = (R) s.gauss();
+    const bool uni  = sp.phsp == MQI_PHSP_UNIFORM;
+    auto       draw = [&]() { return uni ? (R) (2.0f * s.uniform() - 1.0f) : (R) s.gauss(); };
+    R          Ux   = draw();
+    R          Vx   = draw();
+    R          Uy   = draw();
+    R          Vy   = draw();
+    R          Uz   = draw();
     R ke = sp.energy;
     if (!sp.energy_const) {
         R z = (R) s.gauss();

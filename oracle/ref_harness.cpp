@@ -322,16 +322,18 @@ ref_grid_probe(int op, int nx, int ny, int nz, const float* xe, const float* ye,
 // transform) and a transport seed from tmp_rng.
 // mean6/sigma6: x,y,z,x',y',z' (z' mean and sigma ignored by phsp_6d).
 void
-ref_sample_spot(float energy, float sigma_e, int const_energy, const float* mean6, const float* sigma6,
-                const float* corr2, const float* angles4, const float* iso3, uint32_t master_seed,
-                uint32_t spot, int64_t n, float* vtx_out, int32_t* seed_out) {
+ref_sample_spot(float energy, float sigma_e, int const_energy, int uniform, const float* mean6,
+                const float* sigma6, const float* corr2, const float* angles4, const float* iso3,
+                uint32_t master_seed, uint32_t spot, int64_t n, float* vtx_out, int32_t* seed_out) {
     std::array<R, 6> m, s;
     std::array<R, 2> r = { corr2 ? corr2[0] : 0.0f, corr2 ? corr2[1] : 0.0f };
     for (int k = 0; k < 6; ++k) {
         m[k] = mean6[k];
         s[k] = sigma6[k];
     }
-    mqi::phsp_6d<R>     fluence(m, s, r);
+    mqi::phsp_6d<R>         gauss(m, s, r);
+    mqi::phsp_6d_uniform<R> flat(m, s, r);
+    mqi::pdf_Md<R, 6>*      fluence = uniform ? (mqi::pdf_Md<R, 6>*) &flat : (mqi::pdf_Md<R, 6>*) &gauss;
     mqi::pdf_Md<R, 1>*  e;
     if (const_energy)
         e = new mqi::const_1d<R>({ energy }, { sigma_e });
@@ -340,7 +342,7 @@ ref_sample_spot(float energy, float sigma_e, int const_energy, const float* mean
     std::array<R, 4>             ang = { angles4[0], angles4[1], angles4[2], angles4[3] };
     mqi::vec3<R>                 iso(iso3[0], iso3[1], iso3[2]);
     mqi::coordinate_transform<R> pc(ang, iso);
-    mqi::beamlet<R>              bl(e, &fluence);
+    mqi::beamlet<R>              bl(e, fluence);
     bl.set_coordinate_transform(pc);
     std::default_random_engine beam_rng, tmp_rng;
     beam_rng.seed(master_seed + spot);

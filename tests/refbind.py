@@ -22,7 +22,7 @@ c_u64 = ctypes.POINTER(ctypes.c_uint64)
 
 SPOT_DTYPE = np.dtype([
     ("energy", np.float32), ("sigma_energy", np.float32), ("energy_const", np.int32),
-    ("reserved", np.uint32), ("mean", np.float32, 6), ("sigma", np.float32, 6),
+    ("phsp", np.uint32), ("mean", np.float32, 6), ("sigma", np.float32, 6),
     ("corr", np.float32, 2), ("rotation", np.float32, 9), ("translation", np.float32, 3),
     ("histories", np.uint64)])
 assert SPOT_DTYPE.itemsize == 128
@@ -187,8 +187,9 @@ def oracle_run_spots(grid, spots, mask=SC_DOSE, kernel=0, threads=1, prec=32, rn
 
 
 def make_spot(energy, sigma_xy, n, pos=(0.0, 0.0, 0.0), sigma_e=0.0, energy_const=True, rotation=None,
-              translation=(0.0, 0.0, 0.0), sigma_dir=(0.0, 0.0), corr=(0.0, 0.0), z_mean=None):
+              translation=(0.0, 0.0, 0.0), sigma_dir=(0.0, 0.0), corr=(0.0, 0.0), z_mean=None, uniform=False):
     s = np.zeros(1, SPOT_DTYPE)[0]
+    s["phsp"] = 1 if uniform else 0
     s["energy"] = energy
     s["sigma_energy"] = sigma_e
     s["energy_const"] = 1 if energy_const else 0
@@ -202,6 +203,8 @@ def make_spot(energy, sigma_xy, n, pos=(0.0, 0.0, 0.0), sigma_e=0.0, energy_cons
 
 
 def ref_sample_spot(spot, angles, iso, master_seed, spot_index, n, debug=0):
+    """Reference beamlet sampling (phsp_6d, or phsp_6d_uniform when the
+    spot's phsp is MQI_PHSP_UNIFORM) with run_by_spot's seeding."""
     lib = reference(debug)
     vtx = np.zeros((n, 7), np.float32)
     seeds = np.zeros(n, np.int32)
@@ -211,7 +214,7 @@ def ref_sample_spot(spot, angles, iso, master_seed, spot_index, n, debug=0):
     ang = np.ascontiguousarray(angles, np.float32)
     isoa = np.ascontiguousarray(iso, np.float32)
     lib.ref_sample_spot(ctypes.c_float(spot["energy"]), ctypes.c_float(spot["sigma_energy"]),
-                        int(spot["energy_const"]), ptr(mean, c_f), ptr(sig, c_f), ptr(corr, c_f),
+                        int(spot["energy_const"]), int(spot["phsp"]), ptr(mean, c_f), ptr(sig, c_f), ptr(corr, c_f),
                         ptr(ang, c_f), ptr(isoa, c_f), ctypes.c_uint32(master_seed),
                         ctypes.c_uint32(spot_index), ctypes.c_int64(n), ptr(vtx, c_f), ptr(seeds, c_i32))
     return vtx, seeds

@@ -111,3 +111,47 @@ def test_null_context_calls_fail_cleanly():
     assert lib.mqi_destroy(None) in (0, 1)
     assert lib.mqi_zero_scorers(None) == 1
     assert lib.mqi_run(None, None, None) == 1
+
+
+HOST = os.path.join(ROOT, "moquimc_amd", "host")
+
+
+def test_host_layer_rotation_matches_oracle():
+    """mqi_hip::coordinate_rotation (C++ environment layer) equals the
+    oracle's restatement of coordinate_transform, which equals the
+    reference's (spot sampling tests), bit for bit."""
+    import tempfile
+    import refbind as rb
+    angles = [(0, 0, 0, 0), (0, 0, 0, 90), (15, 30, 10, 90), (-20, 45.5, 270, 90), (90, 180, 0, 0)]
+    src = "#include <cstdio>\n#include \"mqi_hip_env.hpp\"\nint main(){float a[4];\n" \
+          "while (std::scanf(\"%f %f %f %f\", a, a+1, a+2, a+3) == 4) {\n" \
+          "auto r = mqi_hip::coordinate_rotation({a[0], a[1], a[2], a[3]});\n" \
+          "for (float v : r) std::printf(\"%a \", v); std::printf(\"\\n\"); } return 0; }\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "rot.cpp")
+        exe = os.path.join(d, "rot")
+        open(c, "w").write(src)
+        subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), "-I", HOST, c, "-o", exe],
+                       check=True)
+        inp = "\n".join(" ".join(str(x) for x in a) for a in angles) + "\n"
+        out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
+    for a, line in zip(angles, out):
+        got = np.array([float.fromhex(t) for t in line.split()], np.float32)
+        want = rb.coordinate_rotation(a)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), a
+
+
+@needs_lib
+def test_host_driver_fails_cleanly_without_device():
+    import torch
+    exe = os.path.join(HOST, "phantom_env")
+    if not os.path.exists(exe):
+        pytest.skip("phantom_env not built (python -m moquimc_amd.build)")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    r = subprocess.run([exe, "--output_prefix", "/nonexistent/", "--nxyz", "4", "4", "4"], capture_output=True,
+                       text=True)
+    assert r.returncode == 1
+    assert "mqi_create" in r.stderr
+    r = subprocess.run([exe, "--nxyz", "4", "4"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--nxyz needs 3 values" in r.stderr

@@ -163,3 +163,46 @@ def test_fresh_context_first_run_binds_work_list():
             assert np.allclose(c.scorer(0), ref[0], rtol=1e-9, atol=0)
     finally:
         c.close()
+
+
+def test_exact_uniform_phase_space_matches_oracle(ctx):
+    """phsp_6d_uniform spots (the phantom driver's beam) through the same
+    verification comparison."""
+    g = phantoms.water_box(30, 30, 160)
+    sp = np.array([rb.make_spot(130.0, 3.0, 400, pos=(0, 0, 0.5), uniform=True),
+                   rb.make_spot(90.0, 2.0, 300, pos=(4, -3, 0.5), energy_const=False, sigma_e=0.9, uniform=True)],
+                  dtype=rb.SPOT_DTYPE)
+    ref, ntr = rb.oracle_run_spots(g, sp, mask=rb.SC_DOSE, kernel=2, prec=32, rng=1, math=1, debug=1, master_seed=7,
+                                   threads=4)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE)
+    res = ctx.run(ctx.config(seed=7, exact=True, batched=True, physics_debug=True))
+    assert res.histories == ntr == 700
+    assert np.allclose(ctx.scorer(0), ref[0], rtol=1e-9, atol=0)
+
+
+def test_cpp_phantom_driver_matches_abi(ctx, tmp_path):
+    """The C++ environment layer (mqi_hip::phantom_env through the phantom
+    command line) and the ctypes path give the same dose for the same plan."""
+    import os
+    import subprocess
+    exe = os.path.join(rb.ROOT, "moquimc_amd", "host", "phantom_env")
+    prefix = str(tmp_path) + "/"
+    r = subprocess.run([exe, "--output_prefix", prefix, "--lxyz", "30", "30", "120", "--pxyz", "0", "0", "-60",
+                        "--nxyz", "30", "30", "120", "--spot_position", "0", "0", "0.5", "--spot_size", "3", "2",
+                        "--spot_energy", "110", "0", "--histories", "600", "--random_seed", "5", "--exact", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    dose_cpp = np.fromfile(prefix + "dose.f64", np.float64)
+    g = phantoms.water_box(30, 30, 120, rho=0.0)
+    hu = np.zeros(30 * 30 * 120, np.int16)
+    ctx.set_grid_hu(g.xe, g.ye, g.ze, hu)
+    sp = np.array([rb.make_spot(110.0, 0.0, 600, pos=(0, 0, 0.5), uniform=True)], dtype=rb.SPOT_DTYPE)
+    sp[0]["sigma"][:2] = (3.0, 2.0)
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE)
+    res = ctx.run(ctx.config(seed=5, exact=True, batched=True, physics_debug=True))
+    assert res.histories == 600
+    assert dose_cpp.shape == ctx.scorer(0).shape
+    assert np.allclose(dose_cpp, ctx.scorer(0), rtol=1e-9, atol=0)

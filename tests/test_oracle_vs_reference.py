@@ -60,10 +60,12 @@ def test_rotated_translated_grid_bit_exact():
 
 
 @needs_ref
+@pytest.mark.parametrize("uniform", [False, True])
 @pytest.mark.parametrize("angles", [(0, 0, 0, 0), (0, 0, 0, 90), (15, 30, 10, 90)])
-def test_spot_sampling_bit_exact(angles):
+def test_spot_sampling_bit_exact(angles, uniform):
+    """phsp_6d and phsp_6d_uniform beamlets with the coordinate transform."""
     sp = rb.make_spot(150.0, 4.0, 500, pos=(2.0, -1.0, 0.5), energy_const=False, sigma_e=1.5,
-                      sigma_dir=(0.002, 0.003), corr=(0.3, -0.2))
+                      sigma_dir=(0.002, 0.003), corr=(0.3, -0.2), uniform=uniform)
     rot = rb.coordinate_rotation(angles)
     sp2 = sp.copy()
     sp2["rotation"] = rot

@@ -106,3 +106,23 @@ def test_philox_spot_sampling_matches_reference_distribution():
         assert abs(arr[:, 1].mean() - 2.0) < 5 * 4.0 / np.sqrt(n)
         assert abs(arr[:, 1].std() / 4.0 - 1) < 0.01
         assert abs(arr[:, 2].mean() + 1.0) < 5 * 4.0 / np.sqrt(n)
+
+
+def test_philox_uniform_phase_space_matches_reference_distribution():
+    """phsp_6d_uniform: x = mean + sigma U(-1, 1): the Philox stream and the
+    reference RNG give the same moments; the support is [mean - sigma, mean + sigma]."""
+    n = 200000
+    sp = rb.make_spot(100.0, 3.0, n, pos=(1.0, -2.0, 0.5), sigma_dir=(0.001, 0.002), corr=(0.4, 0.1), uniform=True)
+    a, _ = rb.oracle_sample_spot(sp, 5, 1, n, rng=0)
+    b, _ = rb.oracle_sample_spot(sp, 5, 1, n, rng=1)
+    for arr in (a, b):
+        x, y = arr[:, 1].astype(np.float64), arr[:, 2].astype(np.float64)
+        assert x.min() >= 1.0 - 3.0 - 1e-5 and x.max() <= 1.0 + 3.0 + 1e-5
+        assert abs(x.mean() - 1.0) < 5 * 3.0 / np.sqrt(3 * n)
+        assert abs(x.std() / (3.0 / np.sqrt(3.0)) - 1) < 0.01
+        assert abs(y.mean() + 2.0) < 5 * 3.0 / np.sqrt(3 * n)
+        assert np.all(arr[:, 0] == np.float32(100.0))
+    for k in range(1, 7):
+        sa, sb = a[:, k].astype(np.float64), b[:, k].astype(np.float64)
+        se = np.sqrt(sa.var() / n + sb.var() / n)
+        assert abs(sa.mean() - sb.mean()) <= 5 * se + 1e-12, k
