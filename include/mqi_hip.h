@@ -60,6 +60,8 @@ typedef struct mqi_run_config {
     int32_t  blocks;           /* workgroups of the persistent launch; 0: fill the device */
     int32_t  waves_per_simd;   /* register budget of the production kernels in waves/SIMD: 2 (compiler's), 3-6, 8; 0: 4 */
     int32_t  fetch_chunk;      /* histories a lane takes per work-queue fetch (batched); 0: 1 */
+    int32_t  refill_lanes;     /* finished lanes a wave gathers before fetching new histories (batched); 0: 1 */
+    int32_t  reserved0;
     int32_t  use_graph;        /* 1: capture the batched round in a hipGraph and replay it */
     float    stat_criterion;   /* > 0: repeat rounds until mean(sd/mean) <= criterion (a fraction) */
     float    stat_threshold;   /* voxels with mean > threshold * max mean enter the criterion */

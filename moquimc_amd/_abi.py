@@ -26,7 +26,8 @@ class RunConfig(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("physics_debug", ctypes.c_int32), ("exact", ctypes.c_int32),
         ("batched", ctypes.c_int32), ("scoring_loops", ctypes.c_int32), ("stack_depth", ctypes.c_int32),
         ("block_size", ctypes.c_int32), ("blocks", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
-        ("fetch_chunk", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+        ("fetch_chunk", ctypes.c_int32), ("refill_lanes", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
         ("stat_criterion", ctypes.c_float), ("stat_threshold", ctypes.c_float), ("max_rounds", ctypes.c_int32),
         ("dij", ctypes.c_int32), ("dij_capacity", ctypes.c_uint64), ("hist_offset", ctypes.c_uint64)]
 
@@ -201,11 +202,11 @@ class Context:
 
     @staticmethod
     def config(seed=1, physics_debug=False, exact=False, batched=True, loops=LOOPS_ONCE, stack_depth=0,
-               block_size=0, blocks=0, waves_per_simd=0, fetch_chunk=0, use_graph=False, stat_criterion=0.0,
-               stat_threshold=0.5, max_rounds=0, dij=False, dij_capacity=0, hist_offset=0):
+               block_size=0, blocks=0, waves_per_simd=0, fetch_chunk=0, refill_lanes=0, use_graph=False,
+               stat_criterion=0.0, stat_threshold=0.5, max_rounds=0, dij=False, dij_capacity=0, hist_offset=0):
         return RunConfig(seed, int(physics_debug), int(exact), int(batched), loops, stack_depth, block_size, blocks,
-                         waves_per_simd, fetch_chunk, int(use_graph), stat_criterion, stat_threshold, max_rounds,
-                         int(dij), dij_capacity, hist_offset)
+                         waves_per_simd, fetch_chunk, refill_lanes, 0, int(use_graph), stat_criterion, stat_threshold,
+                         max_rounds, int(dij), dij_capacity, hist_offset)
 
     def run(self, cfg):
         res = RunResult()

@@ -67,6 +67,7 @@ struct transport_params {
     // (initialised to the range's first history)
     unsigned long long* queue;
     int             fetch_chunk;   // histories a lane takes per fetch (dynamic form), >= 1
+    int             refill_lanes;  // idle lanes a wave gathers before refilling them (dynamic form), >= 1
     const uint64_t* hist_offset;   // device scalar added to every history index (graph-replay safe)
     uint64_t        seed;
     // scorers

@@ -632,9 +632,10 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L) {
     if (TRACE) {
         trace_record(P, L, cnb);
     } else {
-        const uint64_t nxy = (uint64_t) (c.g.nx * c.g.ny);
-        const int      k = (int) (cnb / nxy), j = (int) ((cnb % nxy) / c.g.nx), i = (int) ((cnb % nxy) % c.g.nx);
-        float          vol = c.g.X(i + 1) - c.g.X(i);
+        // get_volume(cnb) (mqi_grid3d.hpp:471-487) goes through cnb2ijk; the
+        // lane's cell indices are that (i, j, k) already
+        const int i = t.cx, j = t.cy, k = t.cz;
+        float     vol = c.g.X(i + 1) - c.g.X(i);
         vol *= c.g.Y(j + 1) - c.g.Y(j);
         vol *= c.g.Z(k + 1) - c.g.Z(k);
         score<DBG>(P, t, cnb, rho, vol, L.dij_item, L.scen_off);
@@ -730,7 +731,20 @@ k_transport(transport_params P) {
     }
 
     for (;;) {
-        if (!L.active && c.sp == 0) {
+        // A lane whose history and secondaries are done takes a new history.
+        // With refill_lanes > 1 (dynamic form) it first waits until that
+        // many lanes of its wave need one, so the wave runs the new-history
+        // path (sampling, work-list search, grid entry) once for all of
+        // them; with no lane left to step it refills at once.
+        const bool need   = !L.active && c.sp == 0;
+        bool       refill = need;
+        if (DYNAMIC && P.refill_lanes > 1) {
+            const unsigned long long nm   = __ballot(need);
+            const unsigned long long busy = __ballot(!need);
+            refill = need && (__popcll(nm) >= (unsigned) P.refill_lanes || busy == 0);
+        }
+        if (need && !refill) continue;
+        if (refill) {
             uint64_t g;
             if (DYNAMIC) {
                 if (s_next >= s_end) {

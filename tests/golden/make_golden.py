@@ -168,8 +168,8 @@ def profiles_fixture(threads):
 
     c1  water 200x200x350 @ 0.5x0.5x1 mm, one 200 MeV spot (sigma 3 mm, no
         energy spread), debug physics, 8 batches x 1.25e6 histories: dose in
-        the box where the mean exceeds 1% of its maximum, its standard error,
-        the integral depth dose and the integral dose.
+        the box above 10% of the maximum (plus 3 voxels), the integral depth
+        dose and the integral dose with their standard errors.
     c2  the slab phantom with the 25-spot plan (70-230 MeV, 1% spread,
         sigma 4 mm), release physics: dose from 8 batches x 1.25e5 histories
         per spot, LETd (numerator and denominator scored without the dose
@@ -182,13 +182,16 @@ def profiles_fixture(threads):
     g = phantoms.water_box(200, 200, 350, dx=0.5, dz=1.0)
     sp = rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5))
     runs = _batches(g, [sp], 1, 1250000, 8, rb.SC_DOSE, threads, 1000)
-    m, se = _mean_se([r[0] for r in runs])
-    d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
-    hot = np.argwhere(d3 > 0.01 * d3.max())
-    lo, hi = hot.min(0), hot.max(0) + 1
+    m, _ = _mean_se([r[0] for r in runs])
+    d3 = m.reshape(350, 200, 200)
+    # the region above 10 % of the maximum plus 3 voxels (the 1 mm search of
+    # the gamma index), as float16 relative to the maximum (error <= 2.5e-4)
+    hot = np.argwhere(d3 >= 0.1 * d3.max())
+    lo = np.maximum(hot.min(0) - 3, 0)
+    hi = np.minimum(hot.max(0) + 4, d3.shape)
     out["c1_box"] = np.concatenate([lo, hi]).astype(np.int32)   # z0 y0 x0 z1 y1 x1
-    out["c1_dose"] = d3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].astype(np.float32)
-    out["c1_dose_se"] = s3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]].astype(np.float16)
+    out["c1_dose_scale"] = np.array(d3.max())
+    out["c1_dose_rel"] = (d3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
     idd = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
     out["c1_idd"], out["c1_idd_se"] = _mean_se(idd)
     tot = np.array([r[0].sum() for r in runs])
@@ -213,8 +216,8 @@ def profiles_fixture(threads):
     runs = _batches(g, spots, 0, 25000, 4, rb.SC_LETD, threads, 3000)
     num = np.sum([r[0] for r in runs], 0).reshape(350, 200, 200)
     den = np.sum([r[1] for r in runs], 0).reshape(350, 200, 200)
-    out["c2_let_num_xz"], out["c2_let_den_xz"] = num[:, 100, :], den[:, 100, :]
-    out["c2_let_num_yz"], out["c2_let_den_yz"] = num[:, :, 100], den[:, :, 100]
+    out["c2_let_num_xz"], out["c2_let_den_xz"] = num[:, 100, :].astype(np.float32), den[:, 100, :].astype(np.float32)
+    out["c2_let_num_yz"], out["c2_let_den_yz"] = num[:, :, 100].astype(np.float32), den[:, :, 100].astype(np.float32)
     out["c2_let_histories"] = np.array(4 * 25000 * len(spots))
     print("c2 done in %.0f s" % (time.time() - t0), flush=True)
     return out

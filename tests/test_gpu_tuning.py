@@ -64,3 +64,55 @@ def test_autotune_picks_a_working_configuration(ctx):
     assert res.histories == 1500 and res.dropped == 0
     with pytest.raises(Exception):
         ctx.autotune(ctx.config(exact=True), sample=0.5)
+
+
+def test_graph_replays_match_eager(ctx):
+    """use_graph: the captured batched pass, replayed several times with eager
+    work in between, gives the eager result every time; a new plan is
+    captured anew."""
+    _setup(ctx)
+    eager = []
+    for off in (0, 5000):
+        ctx.zero_scorers()
+        ctx.run(ctx.config(seed=44, batched=True, exact=True, hist_offset=off))
+        eager.append(ctx.scorer(0))
+    for rep in range(3):
+        for k, off in enumerate((0, 5000)):
+            ctx.zero_scorers()
+            res = ctx.run(ctx.config(seed=44, batched=True, exact=True, use_graph=True, hist_offset=off))
+            assert res.histories == 1500
+            assert np.allclose(ctx.scorer(0), eager[k], rtol=1e-9, atol=0), (rep, k)
+        # eager work between replays
+        ctx.zero_scorers()
+        ctx.run(ctx.config(seed=45, batched=True))
+    # a different plan: recaptured
+    g = phantoms.water_box(20, 20, 100)
+    sp = phantoms.spots([90.0], 700, sigma=2.0)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE)
+    ctx.run(ctx.config(seed=46, batched=True, exact=True))
+    want = ctx.scorer(0)
+    ctx.zero_scorers()
+    res = ctx.run(ctx.config(seed=46, batched=True, exact=True, use_graph=True))
+    assert res.histories == 700
+    assert np.allclose(ctx.scorer(0), want, rtol=1e-9, atol=0)
+
+
+def test_graph_with_stat_rounds(ctx):
+    """Statistical rounds replay the captured pass with a new history offset
+    each round: same scorers as the eager rounds."""
+    g = phantoms.slab_box(24, 24, 140)
+    sp = phantoms.spots([110.0, 130.0], 300, sigma=3.0, sigma_e_frac=0.01, positions=[(0, 0, 0.5), (3, 2, 0.5)])
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE | rb.SC_STAT)
+    out = []
+    for graph in (False, True):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=47, batched=True, exact=True, use_graph=graph, stat_criterion=1e-6,
+                                 max_rounds=3))
+        assert res.rounds == 3
+        out.append([ctx.scorer(k) for k in range(3)])
+    for a, b in zip(*out):
+        assert np.allclose(a, b, rtol=1e-9, atol=0)
