@@ -186,7 +186,7 @@ def main():
         t0 = time.time()
         cfg, best = ctx.autotune(cfg, sample=0.1)
         tuned = {"waves_per_simd": cfg.waves_per_simd, "block_size": cfg.block_size, "fetch_chunk": cfg.fetch_chunk,
-                 "autotune_s": round(time.time() - t0, 2)}
+                 "refill_lanes": cfg.refill_lanes, "autotune_s": round(time.time() - t0, 2)}
         log("autotuned: %s" % tuned)
     for i in range(args.warmup):
         ctx.zero_scorers()

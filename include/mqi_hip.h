@@ -60,7 +60,7 @@ typedef struct mqi_run_config {
     int32_t  blocks;           /* workgroups of the persistent launch; 0: fill the device */
     int32_t  waves_per_simd;   /* register budget of the production kernels in waves/SIMD: 2 (compiler's), 3-6, 8; 0: 4 */
     int32_t  fetch_chunk;      /* histories a lane takes per work-queue fetch (batched); 0: 1 */
-    int32_t  refill_lanes;     /* finished lanes a wave gathers before fetching new histories (batched); 0: 1 */
+    int32_t  refill_lanes;     /* finished lanes a wave gathers before fetching new histories (batched); 0: 4 */
     int32_t  reserved0;
     int32_t  use_graph;        /* 1: capture the batched round in a hipGraph and replay it */
     float    stat_criterion;   /* > 0: repeat rounds until mean(sd/mean) <= criterion (a fraction) */
@@ -133,8 +133,9 @@ int mqi_run(mqi_ctx* ctx, const mqi_run_config* cfg, mqi_run_result* res);
 /* Autotuner (no counterpart in the reference, which fixes 512 threads per
  * block and one history per thread, mqi_tps_env.hpp:1058-1083): times
  * batched passes of the current plan with every spot's histories scaled by
- * `sample` (0 < sample <= 1) over waves_per_simd {3, 4, 5} x block_size
- * {128, 256} x fetch_chunk {1, 2, 4} and writes the fastest into *cfg.  With
+ * `sample` (0 < sample <= 1): waves_per_simd {3, 4, 5} x block_size {128, 256},
+ * then refill_lanes {1, 2, 4, 8}, then fetch_chunk {1, 2, 4} (coordinate
+ * descent), and writes the fastest into *cfg.  With
  * cfg->stack_depth < 0 it also picks the smallest secondary stack (16 .. 512)
  * that drops nothing on the sample (the reference's depth drops silently,
  * mqi_track_stack.hpp:36-41).  The trial passes score into the scorers, which

@@ -337,7 +337,8 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.item_scenario = c->d_item_scen;
     P.queue         = c->d_queue;
     P.fetch_chunk   = cfg.fetch_chunk > 0 ? cfg.fetch_chunk : 1;
-    P.refill_lanes  = cfg.refill_lanes > 0 ? cfg.refill_lanes : 1;
+    // 0: 4, the measured optimum on the 2000-spot CT plan (1 refills at once)
+    P.refill_lanes  = cfg.refill_lanes > 0 ? cfg.refill_lanes : 4;
     P.tracked       = c->d_counters + 1;
     P.drops         = c->d_counters + 2;
     P.steps         = c->d_counters + 3;
@@ -919,26 +920,44 @@ mqi_autotune(mqi_ctx* c, mqi_run_config* cfg, double sample, double* best_ms) {
             if (dropped == 0) break;
         }
     }
-    const int waves[]  = {3, 4, 5};
-    const int blocks[] = {128, 256};
-    const int chunks[] = {1, 2, 4};
-    double    best     = 1e300;
+    // coordinate descent: register budget x workgroup size, then the refill
+    // gathering, then the histories per fetch
+    double         best = 1e300;
     mqi_run_config pick = base;
-    for (int w : waves)
-        for (int b : blocks)
-            for (int ch : chunks) {
-                mqi_run_config t = base;
-                t.waves_per_simd = w;
-                t.block_size     = b;
-                t.fetch_chunk    = ch;
-                if ((r = trial_pass(c, t, &ms, &dropped))) return r;
-                if (ms < best) {
-                    best = ms;
-                    pick = t;
-                }
-            }
+    pick.refill_lanes   = 4;
+    pick.fetch_chunk    = 1;
+    auto try_cfg = [&](const mqi_run_config& t) -> int {
+        int e = trial_pass(c, t, &ms, &dropped);
+        if (e) return e;
+        if (ms < best) {
+            best = ms;
+            pick = t;
+        }
+        return 0;
+    };
+    const mqi_run_config start = pick;
+    for (int w : {3, 4, 5})
+        for (int b : {128, 256}) {
+            mqi_run_config t = start;
+            t.waves_per_simd = w;
+            t.block_size     = b;
+            if ((r = try_cfg(t))) return r;
+        }
+    const mqi_run_config stage2 = pick;
+    for (int k : {1, 2, 8}) {
+        mqi_run_config t = stage2;
+        t.refill_lanes   = k;
+        if ((r = try_cfg(t))) return r;
+    }
+    const mqi_run_config stage3 = pick;
+    for (int ch : {2, 4}) {
+        mqi_run_config t = stage3;
+        t.fetch_chunk    = ch;
+        if ((r = try_cfg(t))) return r;
+    }
     cfg->waves_per_simd = pick.waves_per_simd;
     cfg->block_size     = pick.block_size;
+    cfg->refill_lanes   = pick.refill_lanes;
     cfg->fetch_chunk    = pick.fetch_chunk;
     if (tune_stack) cfg->stack_depth = pick.stack_depth;
     if (best_ms) *best_ms = best;

@@ -142,12 +142,22 @@ def transport_fixture():
 def _batches(g, spots, dbg, per_spot, n_batches, mask, threads, seed0):
     """n_batches independent reference runs of the spot list (per_spot
     histories per spot each); returns the per-batch scorer arrays divided by
-    the histories of the batch."""
+    the histories of the batch.
+
+    Vertices come from the reference's beamlet sampling; the per-history
+    transport seeds do not: run_by_spot draws them as consecutive outputs of
+    a minstd engine (mqi_tps_env.hpp:1537-1550) and the CPU transport seeds
+    each history's minstd engine with one, so history i+1 replays history
+    i's random numbers shifted by one draw and the histories of a batch are
+    strongly correlated (batch means scatter ~25x more than independent
+    histories allow).  Independent seeds from a 64-bit generator keep the
+    reference's per-history algorithm and make the statistics nominal."""
     out = []
     for b in range(n_batches):
         vtx, seeds = [], []
         for k, sp in enumerate(spots):
-            v, sd = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], seed0 + b, k, per_spot, debug=dbg)
+            v, _ = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], seed0 + b, k, per_spot, debug=dbg)
+            sd = np.random.default_rng([seed0 + b, k]).integers(1, 2 ** 31 - 1, size=per_spot).astype(np.int32)
             vtx.append(v)
             seeds.append(sd)
         vtx = np.concatenate(vtx)
@@ -182,8 +192,8 @@ def profiles_fixture(threads):
     g = phantoms.water_box(200, 200, 350, dx=0.5, dz=1.0)
     sp = rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5))
     runs = _batches(g, [sp], 1, 1250000, 8, rb.SC_DOSE, threads, 1000)
-    m, _ = _mean_se([r[0] for r in runs])
-    d3 = m.reshape(350, 200, 200)
+    m, se = _mean_se([r[0] for r in runs])
+    d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
     # the region above 10 % of the maximum plus 3 voxels (the 1 mm search of
     # the gamma index), as float16 relative to the maximum (error <= 2.5e-4)
     hot = np.argwhere(d3 >= 0.1 * d3.max())
@@ -192,6 +202,7 @@ def profiles_fixture(threads):
     out["c1_box"] = np.concatenate([lo, hi]).astype(np.int32)   # z0 y0 x0 z1 y1 x1
     out["c1_dose_scale"] = np.array(d3.max())
     out["c1_dose_rel"] = (d3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
+    out["c1_dose_se_rel"] = (s3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
     idd = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
     out["c1_idd"], out["c1_idd_se"] = _mean_se(idd)
     tot = np.array([r[0].sum() for r in runs])
