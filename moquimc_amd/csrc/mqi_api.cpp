@@ -73,6 +73,11 @@ struct mqi_ctx {
     // counters: next, tracked, drops
     unsigned long long* d_counters = nullptr;   // [1] tracked, [2] dropped, [3] steps
     unsigned long long* d_queue    = nullptr;   // per-XCD work-range counters of dynamic launches
+    // Dose derived from Dose_stat (mqi_run with each scorer once): the kernel
+    // scores Dose_stat only; d_snap holds Dose_stat before the run
+    bool    alias_dose = false;
+    double* d_snap     = nullptr;
+    size_t  snap_n     = 0;
     unsigned int*       d_status   = nullptr;
     uint64_t*           d_hoff     = nullptr;
     float*              d_statmax  = nullptr;
@@ -345,10 +350,14 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.status        = c->d_status;
     P.hist_offset   = c->d_hoff;
     P.seed          = cfg.seed;
-    P.n_scorers     = (int) c->kinds.size();
+    // alias_dose: the Dose scorer is derived from Dose_stat after the run
+    // (the kernel then does one atomic less per step)
+    P.n_scorers = 0;
     for (size_t k = 0; k < c->kinds.size(); ++k) {
-        P.kinds[k] = c->kinds[k];
-        P.out[k]   = c->d_out[k];
+        if (c->alias_dose && k == 0) continue;
+        P.kinds[P.n_scorers] = c->kinds[k];
+        P.out[P.n_scorers]   = c->d_out[k];
+        P.n_scorers++;
     }
     if (cfg.dij) {
         P.kinds[P.n_scorers] = MQI_KIND_DIJ;
@@ -657,6 +666,7 @@ mqi_destroy(mqi_ctx* c) {
     for (auto& p : c->d_out) dfree(p);
     dfree(c->d_counters);
     dfree(c->d_queue);
+    dfree(c->d_snap);
     dfree(c->d_status);
     dfree(c->d_hoff);
     dfree(c->d_statmax);
@@ -830,6 +840,25 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     out.stat          = -1.0;
     const int max_rounds = cfg.max_rounds > 0 ? cfg.max_rounds : 64;
     run_plan  p          = make_plan(c, -1);
+    // Dose and Dose_stat receive the same per-step value except in voxel 0
+    // (excluded from Dose, mqi_roi.hpp): with each scorer scored once, score
+    // Dose_stat only and derive Dose at the end of the run
+    int       s1 = -1, s2 = -1;
+    const bool alias = cfg.scoring_loops == MQI_LOOPS_ONCE && !c->kinds.empty() && c->kinds[0] == MQI_KIND_DOSE &&
+                       stat_index(c, &s1, &s2) == 0;
+    const uint64_t n_all = nvox(c) * (c->per_scen ? (uint64_t) c->n_scen : 1ull);
+    if (alias) {
+        if (c->snap_n != n_all) {
+            if ((r = dalloc(c, &c->d_snap, n_all))) return r;
+            c->snap_n = n_all;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_snap, c->d_out[s1], sizeof(double) * n_all, hipMemcpyDeviceToDevice, c->stream));
+    }
+    struct alias_guard {
+        mqi_ctx* c;
+        ~alias_guard() { c->alias_dose = false; }
+    } ag{c};
+    c->alias_dose = alias;
     uint64_t  round_len  = p.max_item;
     double    ms         = 0;
     int       rounds     = 0;
@@ -847,14 +876,18 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     unsigned int       st     = 0;
     HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost));
-    if (cfg.stat_criterion > 0 && rounds > 1) {
-        // calculate_average_results: scorers 0..n-3 scaled by N_target / N_tracked
-        // (mqi_tps_env.hpp:1430-1476, mqi_variables.hpp:51-66)
-        const double   w = (double) (p.total * 1.0 / (double) cnt[1]);
-        const uint64_t n = nvox(c) * (c->per_scen ? (uint64_t) c->n_scen : 1ull);
-        for (size_t k = 0; k + 2 < c->kinds.size(); ++k) KCHK(c, mqi_aux::scale(c->d_out[k], n, (double) (float) w, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    // calculate_average_results: scorers 0..n-3 scaled by N_target / N_tracked
+    // (mqi_tps_env.hpp:1430-1476, mqi_variables.hpp:51-66)
+    const bool   rescale = cfg.stat_criterion > 0 && rounds > 1;
+    const double w       = rescale ? (double) (float) (double) (p.total * 1.0 / (double) cnt[1]) : 1.0;
+    if (alias) {
+        KCHK(c, mqi_aux::dose_from_stat(c->d_out[0], c->d_out[s1], c->d_snap, n_all, nvox(c), w, c->stream));
     }
+    if (rescale) {
+        for (size_t k = alias ? 1 : 0; k + 2 < c->kinds.size(); ++k)
+            KCHK(c, mqi_aux::scale(c->d_out[k], n_all, w, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     out.histories = cnt[1];
     out.dropped   = cnt[2];
     out.steps     = cnt[3];

@@ -16,6 +16,9 @@ int math_probe(int fn, const double* x, const double* y, double* out, int64_t n,
 int stat_reduce(const double* s1, const double* s2, uint64_t n, double N, float thr, float* d_max, double* d_sum,
                 unsigned long long* d_cnt, hipStream_t s);
 int scale(double* p, uint64_t n, double w, hipStream_t s);
+// dose += w * (stat - snap) outside voxel 0 of each block of nvox
+int dose_from_stat(double* dose, const double* stat, const double* snap, uint64_t n, uint64_t nvox, double w,
+                   hipStream_t s);
 int dij_compact(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
                 double* item_max, uint32_t* counts, hipStream_t s);
 int dij_write(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,

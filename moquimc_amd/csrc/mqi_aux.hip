@@ -164,6 +164,21 @@ stat_reduce(const double* s1, const double* s2, uint64_t n, double N, float thr,
 }
 
 // rescale of scorers by N_target / N_tracked (calculate_average, mqi_variables.hpp:51-66)
+// dose[i] += w * (stat[i] - snap[i]) outside voxel 0 of every scenario block
+__global__ void k_dose_from_stat(double* dose, const double* stat, const double* snap, uint64_t n, uint64_t nvox,
+                                 double w) {
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
+        if (i % nvox != 0) dose[i] += w * (stat[i] - snap[i]);
+}
+
+int
+dose_from_stat(double* dose, const double* stat, const double* snap, uint64_t n, uint64_t nvox, double w,
+               hipStream_t s) {
+    if (n == 0 || nvox == 0) return 0;
+    hipLaunchKernelGGL(k_dose_from_stat, dim3(grid_for(n)), dim3(256), 0, s, dose, stat, snap, n, nvox, w);
+    return (int) hipGetLastError();
+}
+
 __global__ void k_scale(double* p, uint64_t n, double w) {
     for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x)
         p[i] *= w;
