@@ -429,9 +429,32 @@ MQI_DI void dij_insert(const transport_params& P, uint64_t item, uint32_t voxel,
     atomicOr(P.status, MQI_STATUS_DIJ_FULL);
 }
 
+// Deposits of consecutive steps into the same voxel (and scenario) are
+// summed in registers and added to the scorer arrays once, when the lane
+// moves on (each-scorer-once scoring only; the sums differ from per-step
+// atomics only in FP64 summation order).  Scorer slots beyond MQI_DEFER are
+// scored per step.
+#define MQI_DEFER 4
+struct pending {
+    int64_t  cnb;   // voxel of the pending sums, -1: none
+    uint64_t off;   // scenario offset of the pending sums
+    double   v[MQI_DEFER];
+};
+
+MQI_DI void flush(const transport_params& P, pending& A) {
+    if (A.cnb >= 0) {
+        // accumulated slots are positive, +inf or NaN (the per-step path adds
+        // every value that is not <= 0, NaN included); untouched slots are 0
+#pragma unroll
+        for (int s = 0; s < MQI_DEFER; ++s)
+            if (A.v[s] != 0.0) atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, A.v[s]);
+    }
+    A.cnb = -1;
+}
+
 template<bool DBG>
 MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float density, float volume,
-                  uint64_t dij_item, uint64_t scen_off) {
+                  uint64_t dij_item, uint64_t scen_off, pending& A) {
     const int     n   = P.n_scorers;
     const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT
     // Both loops run over the fixed maximum with constant indices so that the
@@ -451,6 +474,29 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             double d = dose_hit<DBG>(t, density, volume);
             v[s]     = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
         }
+    }
+    if (P.loops == MQI_LOOPS_ONCE) {
+        if ((int64_t) cnb != A.cnb || scen_off != A.off) {
+            flush(P, A);
+            A.cnb = (int64_t) cnb;
+            A.off = scen_off;
+#pragma unroll
+            for (int s = 0; s < MQI_DEFER; ++s) A.v[s] = 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < MQI_MAX_SCORERS; ++s) {
+            if (s >= n) continue;
+            const int  k    = P.kinds[s];
+            const bool stat = (k == MQI_KIND_STAT_DOSE || k == MQI_KIND_STAT_DOSE2);
+            if (!(stat ? roi >= 0 : roi > 0) || v[s] <= 0) continue;
+            if (k == MQI_KIND_DIJ)
+                dij_insert(P, dij_item, (uint32_t) cnb, v[s]);
+            else if (s < MQI_DEFER)
+                A.v[s] += v[s];
+            else
+                atomicAdd(P.out[s] + scen_off + cnb, v[s]);
+        }
+        return;
     }
 #pragma unroll
     for (int s = 0; s < MQI_MAX_SCORERS; ++s) {
@@ -491,6 +537,7 @@ struct lane {
     uint64_t scen_off;   // scenario offset into the output arrays
     uint64_t dij_item;   // Dij table of (scenario, spot): scenario * n_spots + spot
     int      tsteps;     // steps of the current track
+    pending  acc;        // deferred deposits of the current voxel
 };
 
 // pop + enter the child grid: transport loop head (mqi_transport.hpp:158-191)
@@ -638,7 +685,7 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L) {
         float     vol = c.g.X(i + 1) - c.g.X(i);
         vol *= c.g.Y(j + 1) - c.g.Y(j);
         vol *= c.g.Z(k + 1) - c.g.Z(k);
-        score<DBG>(P, t, cnb, rho, vol, L.dij_item, L.scen_off);
+        score<DBG>(P, t, cnb, rho, vol, L.dij_item, L.scen_off, L.acc);
     }
     if (!t.stopped) {
         step_axis(t.p1.x, t.d1.x, c.g.xe, c.g.sx, t.cx);
@@ -719,6 +766,8 @@ k_transport(transport_params P) {
     L.nsteps   = 0;
     L.scale    = 1.0f;
     L.scen_off = 0;
+    L.acc.cnb  = -1;
+    L.acc.off  = 0;
     c.g.sx = c.g.sy = c.g.sz = 0.0f;
     unsigned long long done = 0, nstep = 0;
     uint32_t           scen = 0;
@@ -833,6 +882,7 @@ k_transport(transport_params P) {
         step<DBG, TRACE>(P, c, L);
         ++nstep;
     }
+    flush(P, L.acc);
     atomicAdd(P.tracked, done);
     atomicAdd(P.steps, nstep);
     if (c.drops) atomicAdd(P.drops, (unsigned long long) c.drops);
