@@ -81,3 +81,18 @@ def within_2sigma(a, sa, b, sb, mask):
     s = np.sqrt(np.asarray(sa, np.float64) ** 2 + np.asarray(sb, np.float64) ** 2)
     d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
     return float(np.mean(d[mask] <= 2.0 * s[mask]))
+
+
+def two_sigma_expectation(sa, sb, dof_a, dof_b, mask):
+    """Expected fraction of |a - b| <= 2 sqrt(sa^2 + sb^2) when a and b have
+    the same mean and sa, sb are batch standard errors with dof_a, dof_b
+    degrees of freedom: the standardised difference then follows Student's
+    t with the Welch-Satterthwaite degrees of freedom (for 8 batches a side
+    about 8-14, i.e. 0.92-0.94 rather than the normal 0.954)."""
+    from scipy import stats
+    va = np.asarray(sa, np.float64)[mask] ** 2
+    vb = np.asarray(sb, np.float64)[mask] ** 2
+    den = va * va / dof_a + vb * vb / dof_b
+    nu = np.where(den > 0, (va + vb) ** 2 / np.where(den > 0, den, 1.0), 1e6)
+    return float(np.mean(2.0 * stats.t.cdf(2.0, nu) - 1.0))
+

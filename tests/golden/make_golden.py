@@ -173,7 +173,7 @@ def _mean_se(xs):
     return a.mean(0), a.std(0, ddof=1) / np.sqrt(len(xs))
 
 
-def profiles_fixture(threads):
+def profiles_fixture(threads, only=None):
     """High-statistics reference runs for the statistical GPU tests.
 
     c1  water 200x200x350 @ 0.5x0.5x1 mm, one 200 MeV spot (sigma 3 mm, no
@@ -181,40 +181,46 @@ def profiles_fixture(threads):
         the box above 10% of the maximum (plus 3 voxels), the integral depth
         dose and the integral dose with their standard errors.
     c2  the slab phantom with the 25-spot plan (70-230 MeV, 1% spread,
-        sigma 4 mm), release physics: dose from 8 batches x 1.25e5 histories
-        per spot, LETd (numerator and denominator scored without the dose
-        scorer) from 4 batches x 2.5e4 per spot; central xz / yz planes and
-        the integral depth dose.
+        sigma 4 mm), release physics: dose from 8 batches x 6.25e5 histories
+        per spot (central xz / yz planes, integral depth dose); LETd
+        (numerator and denominator scored without the dose scorer) from 4
+        batches x 1.25e5 per spot (per-batch sums per depth over the field,
+        central planes).
+    only="c2" recomputes c2 and keeps the c1 arrays of the existing file.
     Dose is per history (Gy)."""
     from moquimc_amd import workloads
     out = {}
-    t0 = time.time()
-    g = phantoms.water_box(200, 200, 350, dx=0.5, dz=1.0)
-    sp = rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5))
-    runs = _batches(g, [sp], 1, 1250000, 8, rb.SC_DOSE, threads, 1000)
-    m, se = _mean_se([r[0] for r in runs])
-    d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
-    # the region above 10 % of the maximum plus 3 voxels (the 1 mm search of
-    # the gamma index), as float16 relative to the maximum (error <= 2.5e-4)
-    hot = np.argwhere(d3 >= 0.1 * d3.max())
-    lo = np.maximum(hot.min(0) - 3, 0)
-    hi = np.minimum(hot.max(0) + 4, d3.shape)
-    out["c1_box"] = np.concatenate([lo, hi]).astype(np.int32)   # z0 y0 x0 z1 y1 x1
-    out["c1_dose_scale"] = np.array(d3.max())
-    out["c1_dose_rel"] = (d3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
-    out["c1_dose_se_rel"] = (s3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
-    idd = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
-    out["c1_idd"], out["c1_idd_se"] = _mean_se(idd)
-    tot = np.array([r[0].sum() for r in runs])
-    out["c1_total"], out["c1_total_se"] = _mean_se(tot)
-    out["c1_histories"] = np.array(8 * 1250000)
-    print("c1 done in %.0f s" % (time.time() - t0), flush=True)
+    if only == "c2":
+        prev = np.load(os.path.join(HERE, "reference_profiles.npz"))
+        out.update({k: prev[k] for k in prev.files if k.startswith("c1_")})
+    if only != "c2":
+        t0 = time.time()
+        g = phantoms.water_box(200, 200, 350, dx=0.5, dz=1.0)
+        sp = rb.make_spot(200.0, 3.0, 0, pos=(0, 0, 0.5))
+        runs = _batches(g, [sp], 1, 1250000, 8, rb.SC_DOSE, threads, 1000)
+        m, se = _mean_se([r[0] for r in runs])
+        d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
+        # the region above 10 % of the maximum plus 3 voxels (the 1 mm search of
+        # the gamma index), as float16 relative to the maximum (error <= 2.5e-4)
+        hot = np.argwhere(d3 >= 0.1 * d3.max())
+        lo = np.maximum(hot.min(0) - 3, 0)
+        hi = np.minimum(hot.max(0) + 4, d3.shape)
+        out["c1_box"] = np.concatenate([lo, hi]).astype(np.int32)   # z0 y0 x0 z1 y1 x1
+        out["c1_dose_scale"] = np.array(d3.max())
+        out["c1_dose_rel"] = (d3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
+        out["c1_dose_se_rel"] = (s3[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] / d3.max()).astype(np.float16)
+        idd = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+        out["c1_idd"], out["c1_idd_se"] = _mean_se(idd)
+        tot = np.array([r[0].sum() for r in runs])
+        out["c1_total"], out["c1_total_se"] = _mean_se(tot)
+        out["c1_histories"] = np.array(8 * 1250000)
+        print("c1 done in %.0f s" % (time.time() - t0), flush=True)
 
     t0 = time.time()
     w = workloads.slabs()
     g = rb.Grid(w.xe, w.ye, w.ze, w.density)
     spots = [s.copy() for s in w.spots]
-    runs = _batches(g, spots, 0, 125000, 8, rb.SC_DOSE, threads, 2000)
+    runs = _batches(g, spots, 0, 625000, 8, rb.SC_DOSE, threads, 2000)
     m, se = _mean_se([r[0] for r in runs])
     d3, s3 = m.reshape(350, 200, 200), se.reshape(350, 200, 200)
     out["c2_xz"], out["c2_xz_se"] = d3[:, 100, :].astype(np.float32), s3[:, 100, :].astype(np.float32)
@@ -223,13 +229,18 @@ def profiles_fixture(threads):
     out["c2_idd"], out["c2_idd_se"] = _mean_se(idd)
     tot = np.array([r[0].sum() for r in runs])
     out["c2_total"], out["c2_total_se"] = _mean_se(tot)
-    out["c2_histories"] = np.array(8 * 125000 * len(spots))
-    runs = _batches(g, spots, 0, 25000, 4, rb.SC_LETD, threads, 3000)
+    out["c2_histories"] = np.array(8 * 625000 * len(spots))
+    del runs
+    # LETd: per-batch dose-weighted sums per depth over the whole field (numerator
+    # and denominator), and the central planes of the summed batches
+    runs = _batches(g, spots, 0, 125000, 4, rb.SC_LETD, threads, 3000)
+    out["c2_let_num_z"] = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+    out["c2_let_den_z"] = np.array([r[1].reshape(350, -1).sum(1) for r in runs])
     num = np.sum([r[0] for r in runs], 0).reshape(350, 200, 200)
     den = np.sum([r[1] for r in runs], 0).reshape(350, 200, 200)
     out["c2_let_num_xz"], out["c2_let_den_xz"] = num[:, 100, :].astype(np.float32), den[:, 100, :].astype(np.float32)
     out["c2_let_num_yz"], out["c2_let_den_yz"] = num[:, :, 100].astype(np.float32), den[:, :, 100].astype(np.float32)
-    out["c2_let_histories"] = np.array(4 * 25000 * len(spots))
+    out["c2_let_histories"] = np.array(4 * 125000 * len(spots))
     print("c2 done in %.0f s" % (time.time() - t0), flush=True)
     return out
 
@@ -237,12 +248,14 @@ def profiles_fixture(threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", action="store_true")
+    ap.add_argument("--only", choices=["c2"], default=None, help="with --stats: recompute the slab workload only")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     args = ap.parse_args()
     if not rb.have_reference():
         raise SystemExit("build the reference harness first: make -C oracle")
     if args.stats:
-        np.savez_compressed(os.path.join(HERE, "reference_profiles.npz"), **profiles_fixture(args.threads))
+        out = profiles_fixture(args.threads, only=args.only)
+        np.savez_compressed(os.path.join(HERE, "reference_profiles.npz"), **out)
         return
     tables = [np.zeros(600, np.float32) for _ in range(6)] + [np.zeros(3996, np.float32)]
     rb.reference(0).ref_tables(*[rb.ptr(t, rb.c_f) for t in tables])

@@ -55,3 +55,19 @@ def test_within_2sigma():
     frac = dosecmp.within_2sigma(a, np.ones_like(a) / np.sqrt(2) * np.sqrt(2), b, np.ones_like(b), np.ones(a.size, bool))
     # |a - b| ~ N(0, 2): within 2 sqrt(2) with probability 95.45 %
     assert abs(frac - 0.9545) < 0.005
+
+
+def test_two_sigma_expectation_matches_student_t():
+    """Batch means with estimated errors: the fraction within 2 estimated
+    sigma follows the t distribution, which the expectation reproduces."""
+    rng = np.random.default_rng(3)
+    nvox, nb = 20000, 8
+    a = rng.normal(size=(nb, nvox))
+    b = rng.normal(size=(nb, nvox)) * 0.3
+    ma, mb = a.mean(0), b.mean(0)
+    sa, sb = a.std(0, ddof=1) / np.sqrt(nb), b.std(0, ddof=1) / np.sqrt(nb)
+    mask = np.ones(nvox, bool)
+    frac = dosecmp.within_2sigma(ma, sa, mb, sb, mask)
+    want = dosecmp.two_sigma_expectation(sa, sb, nb - 1, nb - 1, mask)
+    assert 0.90 < want < 0.93
+    assert abs(frac - want) < 0.01

@@ -11,8 +11,11 @@ dose distributions are compared:
   gamma 1 % / 1 mm (global, above 10 % of max)   >= 99 %
   R80 of the integral depth dose                 within 0.1 mm
   integral dose                                  within 0.5 %
-  per-voxel |difference| <= 2 sigma (combined)   in >= 93 % of voxels above 10 % of max
-                                                 (95.4 % expected if the physics is the same)
+  per-voxel |difference| <= 2 sigma (combined)   in the fraction of voxels above 10 % of max
+                                                 that equal physics gives (Student t with the
+                                                 batch errors' degrees of freedom, ~92 %), less 2 %
+  integral depth dose                            within 3 combined batch errors at >= 90 % of the
+                                                 depths above 10 % of max
 """
 import os
 
@@ -25,6 +28,14 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_profiles.npz")
 needs_fixture = pytest.mark.skipif(not os.path.exists(GOLD), reason="reference_profiles.npz not generated")
+
+
+def _has(key):
+    return os.path.exists(GOLD) and key in np.load(GOLD).files
+
+
+# the slab arrays of the 5x-statistics generation (with per-batch LETd per depth)
+needs_slab_fixture = pytest.mark.skipif(not _has("c2_let_num_z"), reason="slab reference not regenerated")
 
 
 def _gpu_batches(ctx, w, n_batches, per_spot, mask, seed):
@@ -81,11 +92,18 @@ def test_water_200MeV_against_reference(ctx):
     assert abs(ratio - 1.0) <= 0.005
     mask = rd >= 0.1 * rd.max()
     frac = dosecmp.within_2sigma(gd, gs, rd, rs, mask)
-    print("within 2 sigma: %.4f" % frac)
-    assert frac >= 0.93
+    want = dosecmp.two_sigma_expectation(gs, rs, 7, 7, mask)
+    print("within 2 sigma: %.4f (equal physics: %.4f)" % (frac, want))
+    assert frac >= want - 0.02
+    idd_b = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+    idd_se = idd_b.std(0, ddof=1) / np.sqrt(len(idd_b))
+    keep = ref["c1_idd"] >= 0.1 * ref["c1_idd"].max()
+    ok = np.abs(idd_g - ref["c1_idd"]) <= 3.0 * np.sqrt(idd_se ** 2 + ref["c1_idd_se"] ** 2)
+    print("depth dose within 3 sigma at %.3f of %d depths" % (ok[keep].mean(), keep.sum()))
+    assert ok[keep].mean() >= 0.9
 
 
-@needs_fixture
+@needs_slab_fixture
 def test_slabs_25_spots_against_reference(ctx):
     from moquimc_amd import workloads
     ref = np.load(GOLD)
@@ -96,10 +114,12 @@ def test_slabs_25_spots_against_reference(ctx):
     for name, gd, gs in (("xz", d3[:, 100, :], s3[:, 100, :]), ("yz", d3[:, :, 100], s3[:, :, 100])):
         rd, rs = ref["c2_" + name].astype(np.float64), ref["c2_%s_se" % name].astype(np.float64)
         rate, n = dosecmp.gamma_pass_rate(rd, gd, (1.0, 0.5), dd=0.01, dta=1.0, threshold=0.1, refine=2)
-        frac = dosecmp.within_2sigma(gd, gs, rd, rs, rd >= 0.1 * rd.max())
-        print("plane %s: gamma %.4f of %d, within 2 sigma %.4f" % (name, rate, n, frac))
+        mask = rd >= 0.1 * rd.max()
+        frac = dosecmp.within_2sigma(gd, gs, rd, rs, mask)
+        want = dosecmp.two_sigma_expectation(gs, rs, 7, 7, mask)
+        print("plane %s: gamma %.4f of %d, within 2 sigma %.4f (equal physics: %.4f)" % (name, rate, n, frac, want))
         assert rate >= 0.99
-        assert frac >= 0.93
+        assert frac >= want - 0.02
     depth = _depth(w.ze)
     idd_g = m.reshape(350, -1).sum(1)
     r80_g = dosecmp.r80(depth, idd_g[::-1])
@@ -109,28 +129,40 @@ def test_slabs_25_spots_against_reference(ctx):
     ratio = m.sum() / float(ref["c2_total"])
     print("integral dose ratio %.5f" % ratio)
     assert abs(ratio - 1.0) <= 0.005
+    idd_b = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+    idd_se = idd_b.std(0, ddof=1) / np.sqrt(len(idd_b))
+    keep = ref["c2_idd"] >= 0.1 * ref["c2_idd"].max()
+    ok = np.abs(idd_g - ref["c2_idd"]) <= 3.0 * np.sqrt(idd_se ** 2 + ref["c2_idd_se"] ** 2)
+    print("depth dose within 3 sigma at %.3f of %d depths" % (ok[keep].mean(), keep.sum()))
+    assert ok[keep].mean() >= 0.9
 
 
-@needs_fixture
+@needs_slab_fixture
 def test_slabs_letd_against_reference(ctx):
-    """Dose-averaged LET per depth over the central planes (slices above 10 %
-    of the maximum dose) within 2 % of the reference."""
+    """Dose-averaged LET per depth over the whole field (depths above 10 % of
+    the maximum dose): equal within 3 combined batch standard errors at 90 %
+    of the depths (4 batches a side: t with ~6 dof puts 97 % inside), and
+    the median relative difference below 1 %."""
     from moquimc_amd import workloads
     ref = np.load(GOLD)
     w = workloads.slabs()
     runs = _gpu_batches(ctx, w, 4, 250000, 2, seed=23)
-    num = np.sum([r[0] for r in runs], 0).reshape(350, 200, 200)
-    den = np.sum([r[1] for r in runs], 0).reshape(350, 200, 200)
-    for name, gn, gd in (("xz", num[:, 100, :], den[:, 100, :]), ("yz", num[:, :, 100], den[:, :, 100])):
-        rn, rd = ref["c2_let_num_" + name], ref["c2_let_den_" + name]
-        rden, gden = rd.sum(1), gd.sum(1)
-        keep = rden >= 0.1 * rden.max()
-        let_r = rn.sum(1)[keep] / rden[keep]
-        let_g = gn.sum(1)[keep] / gden[keep]
-        rel = np.abs(let_g / let_r - 1)
-        print("plane %s LETd per depth: max rel diff %.4f, median %.4f" % (name, rel.max(), np.median(rel)))
-        assert np.median(rel) <= 0.01
-        assert rel.max() <= 0.02
+    g_num = np.array([r[0].reshape(350, -1).sum(1) for r in runs])
+    g_den = np.array([r[1].reshape(350, -1).sum(1) for r in runs])
+    r_num, r_den = ref["c2_let_num_z"], ref["c2_let_den_z"]
+    keep = r_den.sum(0) >= 0.1 * r_den.sum(0).max()
+    lg_b = g_num[:, keep] / g_den[:, keep]
+    lr_b = r_num[:, keep] / r_den[:, keep]
+    lg = g_num[:, keep].sum(0) / g_den[:, keep].sum(0)
+    lr = r_num[:, keep].sum(0) / r_den[:, keep].sum(0)
+    sg = lg_b.std(0, ddof=1) / np.sqrt(len(lg_b))
+    sr = lr_b.std(0, ddof=1) / np.sqrt(len(lr_b))
+    within = np.abs(lg - lr) <= 3.0 * np.sqrt(sg ** 2 + sr ** 2)
+    rel = np.abs(lg / lr - 1)
+    print("LETd per depth: %d depths, within 3 sigma %.3f, median rel diff %.4f, max %.4f" % (
+        keep.sum(), within.mean(), np.median(rel), rel.max()))
+    assert within.mean() >= 0.9
+    assert np.median(rel) <= 0.01
 
 
 FP64 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_fp64_idd.npz")
