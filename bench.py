@@ -158,6 +158,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--exact", action="store_true", help="time the verification kernels instead")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters instead of the autotuner's")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the Dij workload and its products")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the reference kernel of the HBM counter runs)")
     args = ap.parse_args()
@@ -168,6 +169,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.adjoint_only:
+        from moquimc_amd import _abi
+        print(json.dumps(adjoint_bench(_abi.Context)), flush=True)
         return
 
     from moquimc_amd import _abi, workloads
