@@ -183,7 +183,9 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
  * D^T g).  x: float32 device vector; y: float64 if y_double else float32;
  * FP64 accumulation in a fixed order (results do not depend on scheduling).
  * stream: a hipStream_t or NULL for the context's stream.  The call only
- * enqueues work: y is ready when that stream reaches it. */
+ * enqueues work: y is ready when that stream reaches it.  Products of the same
+ * scenario share a device staging buffer (the compacted x of D^T g): do not
+ * run two of them concurrently on different streams. */
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
 

@@ -34,6 +34,9 @@ int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tm
 int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
                  const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
                  void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
+// D^T g on compacted x: xc = x[nzvox]; ccol = rank[col]
+int gather_x(const float* x, const int32_t* nzvox, int64_t n, float* xc, hipStream_t s);
+int renumber(const int32_t* col, int64_t nnz, const int32_t* rank, int32_t* ccol, hipStream_t s);
 int csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t s);
 int csr_transpose_scatter(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
                           unsigned long long* cursor, int32_t* tcol, float* tval, hipStream_t s);

@@ -309,13 +309,9 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
                                                      int accumulate) {
     __shared__ double buf[MQI_SEG_SHORT];
     const int lane = threadIdx.x;
-    // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
-    // walks a contiguous run of segments (neighbouring segments gather the same
-    // lines of x, which then stay in that XCD's L2).  gridDim.x is a multiple
-    // of 8, so the remap is a permutation of [0, gridDim.x).
-    const int64_t per_xcd = gridDim.x >> 3;
-    const int64_t first   = (int64_t) (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    for (int64_t s = first; s < n_seg; s += gridDim.x) {
+    // plain grid-stride walk: a remap giving each XCD a contiguous run of
+    // segments measured ~7 % slower on the 5000-spot Dij
+    for (int64_t s = blockIdx.x; s < n_seg; s += gridDim.x) {
         const uint32_t info = seg_info[s];
         const int64_t  kb = seg_k[s], ke = seg_k[s + 1];
         if (info >> 31) {
@@ -392,7 +388,7 @@ csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const 
              const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
              void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
     if (n_seg <= 0) return 0;
-    const int blocks = (int) (((n_seg < 262144 ? n_seg : 262144) + 7) & ~(int64_t) 7);
+    const int blocks = (int) (n_seg < 262144 ? n_seg : 262144);
     if (y_is_double)
         hipLaunchKernelGGL(k_csr_adaptive<double>, dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k, seg_info,
                            n_seg, part, (double*) y, alpha, accumulate);
@@ -408,6 +404,32 @@ csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const 
             hipLaunchKernelGGL(k_csr_split_rows<float>, dim3(b), dim3(256), 0, s, seg_info, split_seg, n_split, part,
                                (float*) y, alpha, accumulate);
     }
+    return (int) hipGetLastError();
+}
+
+// Compacted x for D^T g: xc[i] = x[nzvox[i]] (nzvox: the voxels that appear
+// in some row, ascending); the rows' columns renumbered into that list.
+__global__ void k_gather_x(const float* __restrict__ x, const int32_t* __restrict__ nzvox, int64_t n,
+                           float* __restrict__ xc) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        xc[i] = x[nzvox[i]];
+}
+__global__ void k_renumber(const int32_t* __restrict__ col, int64_t nnz, const int32_t* __restrict__ rank,
+                           int32_t* __restrict__ ccol) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t) gridDim.x * blockDim.x)
+        ccol[i] = rank[col[i]];
+}
+
+int
+gather_x(const float* x, const int32_t* nzvox, int64_t n, float* xc, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_gather_x, dim3(grid_for((uint64_t) n)), dim3(256), 0, s, x, nzvox, n, xc);
+    return (int) hipGetLastError();
+}
+int
+renumber(const int32_t* col, int64_t nnz, const int32_t* rank, int32_t* ccol, hipStream_t s) {
+    if (nnz <= 0) return 0;
+    hipLaunchKernelGGL(k_renumber, dim3(grid_for((uint64_t) nnz)), dim3(256), 0, s, col, nnz, rank, ccol);
     return (int) hipGetLastError();
 }
 
