@@ -148,6 +148,70 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
 
 
+def robust_bench(ctx_cls, reps=5):
+    """The 2000-spot CT plan x 9 robust scenarios (nominal 2x histories) in
+    one launch with the Dij scorer (tables of 4096 sqrt(histories) slots per
+    scenario and spot); 9 CSR matrices; worst-case quadratic objective and
+    its gradient through the autograd layer, checked against an FP64 CPU
+    gradient (scipy) of the worst scenario."""
+    import scipy.sparse as sps
+    import torch
+
+    from moquimc_amd import dij, workloads
+    w = workloads.robust()
+    ctx = ctx_cls(0)
+    w.load(ctx)
+    log("robust: %s, %d histories per scenario unit" % (w.name, w.histories))
+    t0 = time.time()
+    res = ctx.run(ctx.config(seed=13, batched=True, dij=True, dij_slots_sqrt=4096))
+    t_tr = time.time() - t0
+    log("robust: transport %.1f s, %d histories, status %d" % (t_tr, res.histories, res.status))
+    t0 = time.time()
+    ctx.dij_build(1e-4)
+    t_build = time.time() - t0
+    log("robust: Dij build %.1f s" % t_build)
+    budgets = w.scenarios[2]
+    # per unit history budget: every scenario on the dose scale of budget 1
+    mats = [dij.DijMatrix(ctx, s, scale=1.0 / float(budgets[s])) for s in range(len(budgets))]
+    dev = torch.device("cuda", 0)
+    n_vox = mats[0].n_voxels
+    target = torch.from_numpy(workloads.target_prescription(w.xe, w.ye, w.ze, radius=30.0, centre=(0.0, 0.0, 0.0),
+                                                            dose=1.0)).to(dev)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    wt = (torch.rand(mats[0].n_spots, generator=gen) + 0.5).to(dev)
+    with torch.no_grad():
+        d0 = mats[0].matvec(wt)
+        scale = d0[target > 0].mean()
+    presc = target.double() * scale
+    ms = []
+    for _ in range(reps + 1):
+        wv = wt.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f, vals = dij.worst_case_objective(mats, wv, presc)
+        f.backward()
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    k = int(torch.argmax(vals).item())
+    log("robust: objective+gradient %.2f ms, worst scenario %d; FP64 CPU check" % (float(np.median(ms[1:])), k))
+    indptr, idx, val, shape = ctx.dij_csr(k)
+    D = sps.csr_matrix((val.astype(np.float64) / float(budgets[k]), idx, indptr), shape=shape)   # spots x voxels
+    wn = wt.double().cpu().numpy()
+    r = D.T @ wn - presc.cpu().numpy()
+    g_ref = 2.0 * (D @ r) / n_vox
+    g = wv.grad.double().cpu().numpy()
+    err = float(np.abs(g - g_ref).max() / np.abs(g_ref).max())
+    out = {"robust_workload": w.name, "robust_histories": int(res.histories),
+           "robust_transport_s": round(t_tr, 2), "robust_kernel_s": round(res.kernel_ms / 1e3, 2),
+           "robust_histories_per_s": round(res.histories / (res.kernel_ms / 1e3), 1),
+           "robust_dij_build_s": round(t_build, 2), "robust_nnz": [int(m.nnz) for m in mats],
+           "robust_objective_grad_ms": round(float(np.median(ms[1:])), 2), "robust_worst_scenario": k,
+           "robust_objective_per_scenario": [float("%.4g" % v) for v in vals.detach().tolist()],
+           "robust_grad_rel_err_fp64": err}
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +220,8 @@ def main():
     ap.add_argument("--plain-passes", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-robust", action="store_true", help="skip the 9-scenario robust Dij leg")
+    ap.add_argument("--robust-only", action="store_true", help="only the 9-scenario robust Dij leg")
     ap.add_argument("--exact", action="store_true", help="time the verification kernels instead")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters instead of the autotuner's")
     ap.add_argument("--adjoint-only", action="store_true", help="only the Dij workload and its products")
@@ -169,6 +235,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.robust_only:
+        from moquimc_amd import _abi
+        print(json.dumps(robust_bench(_abi.Context)), flush=True)
         return
     if args.adjoint_only:
         from moquimc_amd import _abi
@@ -247,6 +317,11 @@ def main():
             out.update(adjoint_bench(_abi.Context))
         except Exception as e:  # reported, not hidden
             out["adjoint_error"] = repr(e)
+    if not args.no_robust:
+        try:
+            out.update(robust_bench(_abi.Context))
+        except Exception as e:  # reported, not hidden
+            out["robust_error"] = repr(e)
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         try:

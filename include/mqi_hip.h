@@ -61,13 +61,15 @@ typedef struct mqi_run_config {
     int32_t  waves_per_simd;   /* register budget of the production kernels in waves/SIMD: 2 (compiler's), 3-6, 8; 0: 4 */
     int32_t  fetch_chunk;      /* histories a lane takes per work-queue fetch (batched); 0: 1 */
     int32_t  refill_lanes;     /* finished lanes a wave gathers before fetching new histories (batched); 0: 4 */
-    int32_t  reserved0;
+    int32_t  dij_slots_sqrt;   /* > 0: the Dij table of a (scenario, spot) with h histories gets
+                                  ceil(dij_slots_sqrt * sqrt(h)) slots (at least 1024, at most dij_capacity
+                                  or 2^24); 0: every table gets dij_capacity slots */
     int32_t  use_graph;        /* 1: capture the batched round in a hipGraph and replay it */
     float    stat_criterion;   /* > 0: repeat rounds until mean(sd/mean) <= criterion (a fraction) */
     float    stat_threshold;   /* voxels with mean > threshold * max mean enter the criterion */
     int32_t  max_rounds;       /* cap on statistical rounds (0: 64) */
     int32_t  dij;              /* 1: score the per-spot dose-influence tables */
-    uint64_t dij_capacity;     /* hash slots per (scenario, spot); 0: 131072 */
+    uint64_t dij_capacity;     /* hash slots per (scenario, spot) (the cap when dij_slots_sqrt > 0); 0: 131072 */
     uint64_t hist_offset;      /* added to every history index (independent repeat runs) */
 } mqi_run_config;
 
@@ -114,8 +116,10 @@ int mqi_set_spots(mqi_ctx* ctx, const mqi_spot* spots, int n_spots);
 
 /* Robust scenarios in one launch: per scenario an edge shift (mm, x y z) and
  * a density scale (the XShift/YShift/ZShift and DensityScaling knobs,
- * mqi_tps_env.hpp:475-477, 766-769).  n = 0 restores the single nominal scenario. */
-int mqi_set_scenarios(mqi_ctx* ctx, int n, const float* shift_xyz, const float* density_scale);
+ * mqi_tps_env.hpp:475-477, 766-769), and optionally a history budget factor
+ * (each spot's histories times budget[s], rounded; NULL: 1).  n = 0 restores
+ * the single nominal scenario. */
+int mqi_set_scenarios(mqi_ctx* ctx, int n, const float* shift_xyz, const float* density_scale, const float* budget);
 
 /* Scorers (MQI_SCORE_* bits of mqi_hip_types.h), in the reference's order
  * Dose, LETd numerator, LETd denominator, Dose_stat, DoseSquare_stat

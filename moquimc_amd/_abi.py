@@ -26,7 +26,7 @@ class RunConfig(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("physics_debug", ctypes.c_int32), ("exact", ctypes.c_int32),
         ("batched", ctypes.c_int32), ("scoring_loops", ctypes.c_int32), ("stack_depth", ctypes.c_int32),
         ("block_size", ctypes.c_int32), ("blocks", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
-        ("fetch_chunk", ctypes.c_int32), ("refill_lanes", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("fetch_chunk", ctypes.c_int32), ("refill_lanes", ctypes.c_int32), ("dij_slots_sqrt", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("stat_criterion", ctypes.c_float), ("stat_threshold", ctypes.c_float), ("max_rounds", ctypes.c_int32),
         ("dij", ctypes.c_int32), ("dij_capacity", ctypes.c_uint64), ("hist_offset", ctypes.c_uint64)]
@@ -67,7 +67,7 @@ def lib():
             "mqi_set_grid_hu": (i, [vp, i, i, i, vp, vp, vp, vp, f, vp, vp]),
             "mqi_get_density": (i, [vp, vp]),
             "mqi_set_spots": (i, [vp, vp, i]),
-            "mqi_set_scenarios": (i, [vp, i, vp, vp]),
+            "mqi_set_scenarios": (i, [vp, i, vp, vp, vp]),
             "mqi_set_scorers": (i, [vp, i, i]),
             "mqi_zero_scorers": (i, [vp]),
             "mqi_scorer_count": (i, [vp]),
@@ -180,15 +180,19 @@ class Context:
         self._chk(self._L.mqi_set_spots(self.h, _p(spots), len(spots)), "mqi_set_spots")
         self.n_spots = len(spots)
 
-    def set_scenarios(self, shifts=None, scales=None):
+    def set_scenarios(self, shifts=None, scales=None, budgets=None):
+        """Robust scenarios: shifts (n, 3) mm, density scales (n), optional
+        history budget factors (n); no arguments: the nominal scenario."""
         if shifts is None:
-            self._chk(self._L.mqi_set_scenarios(self.h, 0, None, None), "mqi_set_scenarios")
+            self._chk(self._L.mqi_set_scenarios(self.h, 0, None, None, None), "mqi_set_scenarios")
             self.n_scen = 1
             return
         shifts = np.ascontiguousarray(shifts, np.float32).reshape(-1, 3)
         scales = np.ascontiguousarray(scales, np.float32).reshape(-1)
-        assert len(shifts) == len(scales)
-        self._chk(self._L.mqi_set_scenarios(self.h, len(scales), _p(shifts), _p(scales)), "mqi_set_scenarios")
+        b = None if budgets is None else np.ascontiguousarray(budgets, np.float32).reshape(-1)
+        if len(shifts) != len(scales) or (b is not None and len(b) != len(scales)):
+            raise ValueError("scenario arrays differ in length")
+        self._chk(self._L.mqi_set_scenarios(self.h, len(scales), _p(shifts), _p(scales), _p(b)), "mqi_set_scenarios")
         self.n_scen = len(scales)
 
     def set_scorers(self, mask, per_scenario=False):
@@ -203,10 +207,11 @@ class Context:
     @staticmethod
     def config(seed=1, physics_debug=False, exact=False, batched=True, loops=LOOPS_ONCE, stack_depth=0,
                block_size=0, blocks=0, waves_per_simd=0, fetch_chunk=0, refill_lanes=0, use_graph=False,
-               stat_criterion=0.0, stat_threshold=0.5, max_rounds=0, dij=False, dij_capacity=0, hist_offset=0):
+               stat_criterion=0.0, stat_threshold=0.5, max_rounds=0, dij=False, dij_capacity=0, hist_offset=0,
+               dij_slots_sqrt=0):
         return RunConfig(seed, int(physics_debug), int(exact), int(batched), loops, stack_depth, block_size, blocks,
-                         waves_per_simd, fetch_chunk, refill_lanes, 0, int(use_graph), stat_criterion, stat_threshold,
-                         max_rounds, int(dij), dij_capacity, hist_offset)
+                         waves_per_simd, fetch_chunk, refill_lanes, dij_slots_sqrt, int(use_graph), stat_criterion,
+                         stat_threshold, max_rounds, int(dij), dij_capacity, hist_offset)
 
     def run(self, cfg):
         res = RunResult()

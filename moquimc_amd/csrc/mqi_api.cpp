@@ -68,6 +68,7 @@ struct mqi_ctx {
     std::vector<mqi_spot> spots;
     mqi_spot*             d_spots = nullptr;
     int                   n_scen  = 1;
+    std::vector<double>   scen_budget;   // histories factor per scenario (empty: 1)
     bool                  scen_set = false;   // mqi_set_scenarios gave explicit shifts / scales
     float *               d_shift = nullptr, *d_scale = nullptr;
 
@@ -101,10 +102,12 @@ struct mqi_ctx {
     size_t    items_cap   = 0;
 
     // Dij
-    uint32_t*            d_dij_keys = nullptr;
-    double*              d_dij_vals = nullptr;
-    uint64_t             dij_cap    = 0;
-    size_t               dij_items  = 0;
+    uint32_t*             d_dij_keys  = nullptr;
+    double*               d_dij_vals  = nullptr;
+    uint64_t*             d_dij_off   = nullptr;   // per-item slot offsets (n_items + 1)
+    uint64_t              dij_alloc   = 0;         // slots allocated
+    size_t                dij_items   = 0;
+    std::vector<uint64_t> dij_off;                 // host copy of the offsets of the last run
     std::vector<csr_dev> dij;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -319,6 +322,7 @@ make_plan(const mqi_ctx* c, int only_spot) {
         for (size_t j = 0; j < c->spots.size(); ++j) {
             if (only_spot >= 0 && (int) j != only_spot) continue;
             uint64_t h = c->spots[j].histories;
+            if (!c->scen_budget.empty()) h = (uint64_t) std::llround((double) h * c->scen_budget[s]);
             if (h == 0) continue;
             p.total += h;
             p.max_item = std::max(p.max_item, h);
@@ -374,7 +378,7 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
         P.n_scorers++;
         P.dij_keys     = c->d_dij_keys;
         P.dij_vals     = c->d_dij_vals;
-        P.dij_capacity = c->dij_cap;
+        P.dij_off      = c->d_dij_off;
     }
     P.out_scenario_stride = c->per_scen ? nvox(c) : 0;
     P.loops               = cfg.scoring_loops;
@@ -427,6 +431,9 @@ check_tuning(mqi_ctx* c, const mqi_run_config& cfg) {
         return fail(c, MQI_E_ARG, "block_size must be 64, 128, 192 or 256");
     if (cfg.blocks < 0) return fail(c, MQI_E_ARG, "blocks must be >= 0");
     if (cfg.stack_depth < 0 || cfg.stack_depth > 4096) return fail(c, MQI_E_ARG, "stack_depth must be in [0, 4096]");
+    if (cfg.dij_slots_sqrt < 0 || cfg.dij_slots_sqrt > (1 << 20))
+        return fail(c, MQI_E_ARG, "dij_slots_sqrt must be in [0, 2^20]");
+    if (cfg.dij_capacity > (1ull << 32)) return fail(c, MQI_E_ARG, "dij_capacity must be at most 2^32");
     return 0;
 }
 
@@ -589,17 +596,42 @@ ready(mqi_ctx* c) {
 
 int
 alloc_dij(mqi_ctx* c, const mqi_run_config& cfg) {
-    const uint64_t cap   = cfg.dij_capacity ? cfg.dij_capacity : 131072;
-    const size_t   items = (size_t) c->n_scen * c->spots.size();
+    // items are indexed scenario * n_spots + spot (every spot, including
+    // empty ones); each owns a contiguous open-addressing table
+    const size_t n_spots = c->spots.size();
+    const size_t items   = (size_t) c->n_scen * n_spots;
+    std::vector<uint64_t> off(items + 1, 0);
+    for (int s = 0; s < c->n_scen; ++s)
+        for (size_t j = 0; j < n_spots; ++j) {
+            uint64_t cap;
+            if (cfg.dij_slots_sqrt > 0) {
+                uint64_t h = c->spots[j].histories;
+                if (!c->scen_budget.empty()) h = (uint64_t) std::llround((double) h * c->scen_budget[s]);
+                const uint64_t hi = cfg.dij_capacity ? cfg.dij_capacity : (1ull << 24);
+                cap = (uint64_t) std::ceil((double) cfg.dij_slots_sqrt * std::sqrt((double) h));
+                cap = std::min(hi, std::max<uint64_t>(cap, 1024));
+            } else {
+                cap = cfg.dij_capacity ? cfg.dij_capacity : 131072;
+            }
+            const size_t i = (size_t) s * n_spots + j;
+            off[i + 1]     = off[i] + cap;
+        }
+    const uint64_t total = off[items];
     int            r;
-    if (cap != c->dij_cap || items != c->dij_items) {
-        if ((r = dalloc(c, &c->d_dij_keys, cap * items)) || (r = dalloc(c, &c->d_dij_vals, cap * items))) return r;
-        c->dij_cap   = cap;
-        c->dij_items = items;
+    if (total > c->dij_alloc || !c->d_dij_keys) {
+        if ((r = dalloc(c, &c->d_dij_keys, total)) || (r = dalloc(c, &c->d_dij_vals, total))) {
+            c->dij_alloc = 0;
+            return r;
+        }
+        c->dij_alloc = total;
     }
-    // items are indexed scenario * n_spots + spot (every spot, including empty ones)
-    KCHK(c, mqi_aux::fill_u32(c->d_dij_keys, 0xffffffffu, cap * items, c->stream));
-    KCHK(c, mqi_aux::fill_u64(c->d_dij_vals, 0ull, cap * items, c->stream));
+    if (items + 1 > c->dij_off.size() || !c->d_dij_off)
+        if ((r = dalloc(c, &c->d_dij_off, items + 1))) return r;
+    HIPCHK(c, hipMemcpy(c->d_dij_off, off.data(), sizeof(uint64_t) * (items + 1), hipMemcpyHostToDevice));
+    c->dij_off   = std::move(off);
+    c->dij_items = items;
+    KCHK(c, mqi_aux::fill_u32(c->d_dij_keys, 0xffffffffu, total, c->stream));
+    KCHK(c, mqi_aux::fill_u64(c->d_dij_vals, 0ull, total, c->stream));
     return 0;
 }
 
@@ -687,6 +719,7 @@ mqi_destroy(mqi_ctx* c) {
     dfree(c->d_item_scen);
     dfree(c->d_dij_keys);
     dfree(c->d_dij_vals);
+    dfree(c->d_dij_off);
     free_dij(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -767,14 +800,20 @@ mqi_set_spots(mqi_ctx* c, const mqi_spot* spots, int n) {
 }
 
 int
-mqi_set_scenarios(mqi_ctx* c, int n, const float* shift, const float* scale) {
+mqi_set_scenarios(mqi_ctx* c, int n, const float* shift, const float* scale, const float* budget) {
     if (!c || n < 0) return MQI_E_ARG;
     hipSetDevice(c->device);
+    c->scen_budget.clear();
     if (n == 0) {
         c->n_scen   = 1;
         c->scen_set = false;
     } else {
         if (!shift || !scale) return fail(c, MQI_E_ARG, "scenario arrays are NULL");
+        if (budget) {
+            for (int k = 0; k < n; ++k)
+                if (!(budget[k] >= 0.0f) || budget[k] > 1e6f) return fail(c, MQI_E_ARG, "bad history budget");
+            c->scen_budget.assign(budget, budget + n);
+        }
         int r;
         if ((r = dalloc(c, &c->d_shift, 3 * (size_t) n)) || (r = dalloc(c, &c->d_scale, (size_t) n))) return r;
         HIPCHK(c, hipMemcpy(c->d_shift, shift, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
@@ -905,8 +944,8 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     out.kernel_ms = ms;
     if (res) *res = out;
     if (st & MQI_STATUS_DIJ_FULL)
-        return fail(c, MQI_E_CAPACITY, "a Dij table overflowed: raise dij_capacity above %llu",
-                    (unsigned long long) c->dij_cap);
+        return fail(c, MQI_E_CAPACITY, "a Dij table overflowed: raise dij_capacity%s",
+                    cfg.dij_slots_sqrt > 0 ? " or dij_slots_sqrt" : "");
     return 0;
 }
 
@@ -1168,6 +1207,8 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
     int r = ready(c);
     if (r) return r;
     if (!c->d_dij_keys) return fail(c, MQI_E_STATE, "no Dij tables: run with cfg.dij = 1 first");
+    if (c->dij_items != (size_t) c->n_scen * c->spots.size())
+        return fail(c, MQI_E_STATE, "the spots or scenarios changed since the run that scored the Dij tables");
     hipSetDevice(c->device);
     free_dij(c);
     const uint32_t n_items = (uint32_t) c->dij_items;
@@ -1175,7 +1216,8 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
     double*        d_max   = nullptr;
     uint32_t*      d_cnt   = nullptr;
     if ((r = dalloc(c, &d_max, n_items)) || (r = dalloc(c, &d_cnt, n_items))) return r;
-    KCHK(c, mqi_aux::dij_compact(c->d_dij_keys, c->d_dij_vals, c->dij_cap, n_items, rel_thr, d_max, d_cnt, c->stream));
+    KCHK(c, mqi_aux::dij_compact(c->d_dij_keys, c->d_dij_vals, c->d_dij_off, n_items, rel_thr, d_max, d_cnt,
+                                 c->stream));
     std::vector<uint32_t> cnt(n_items);
     HIPCHK(c, hipMemcpy(cnt.data(), d_cnt, sizeof(uint32_t) * n_items, hipMemcpyDeviceToHost));
     c->dij.resize(c->n_scen);
@@ -1191,8 +1233,7 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             return r;
         HIPCHK(c, hipMemcpy(m.indptr, indptr.data(), sizeof(int64_t) * (n_spots + 1), hipMemcpyHostToDevice));
         if ((r = build_plan(c, indptr, m.plan))) return r;
-        KCHK(c, mqi_aux::dij_write(c->d_dij_keys + (size_t) s * n_spots * c->dij_cap,
-                                   c->d_dij_vals + (size_t) s * n_spots * c->dij_cap, c->dij_cap, n_spots, rel_thr,
+        KCHK(c, mqi_aux::dij_write(c->d_dij_keys, c->d_dij_vals, c->d_dij_off + (size_t) s * n_spots, n_spots, rel_thr,
                                    d_max + (size_t) s * n_spots, m.indptr, m.col, m.val, c->stream));
         if (m.nnz > 0) {
             // sort every row by voxel

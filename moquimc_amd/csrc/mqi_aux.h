@@ -19,9 +19,9 @@ int scale(double* p, uint64_t n, double w, hipStream_t s);
 // dose += w * (stat - snap) outside voxel 0 of each block of nvox
 int dose_from_stat(double* dose, const double* stat, const double* snap, uint64_t n, uint64_t nvox, double w,
                    hipStream_t s);
-int dij_compact(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+int dij_compact(const uint32_t* keys, const double* vals, const uint64_t* off, uint32_t n_items, double rel_thr,
                 double* item_max, uint32_t* counts, hipStream_t s);
-int dij_write(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+int dij_write(const uint32_t* keys, const double* vals, const uint64_t* off, uint32_t n_items, double rel_thr,
               const double* item_max, const int64_t* row_start, int32_t* cols, float* out_vals, hipStream_t s);
 int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, int64_t nnz, int n_seg,
                    const int64_t* seg_begin, const int64_t* seg_end, void* scratch, size_t* scratch_bytes,

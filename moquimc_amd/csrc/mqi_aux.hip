@@ -191,14 +191,16 @@ scale(double* p, uint64_t n, double w, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ Dij compaction
-// One workgroup per item (spot of a scenario) scans its hash table.
+// One workgroup per item (spot of a scenario) scans its hash table, slots
+// [off[item], off[item + 1]) of keys/vals.
 // Pass A: per-item maximum.  Pass B: count entries >= rel_thr * max.
 // Pass C: write (voxel, value) in slot order into the CSR row; rows are then
 // sorted by voxel with a segmented radix sort.
-__global__ void k_dij_max(const uint32_t* keys, const double* vals, uint64_t cap, double* item_max) {
-    const uint32_t item = blockIdx.x;
-    const uint32_t* k   = keys + (uint64_t) item * cap;
-    const double*   v   = vals + (uint64_t) item * cap;
+__global__ void k_dij_max(const uint32_t* keys, const double* vals, const uint64_t* off, double* item_max) {
+    const uint32_t  item = blockIdx.x;
+    const uint64_t  cap  = off[item + 1] - off[item];
+    const uint32_t* k    = keys + off[item];
+    const double*   v    = vals + off[item];
     double          m   = 0.0;
     for (uint64_t i = threadIdx.x; i < cap; i += blockDim.x)
         if (k[i] != 0xffffffffu && v[i] > m) m = v[i];
@@ -208,11 +210,12 @@ __global__ void k_dij_max(const uint32_t* keys, const double* vals, uint64_t cap
     if (threadIdx.x == 0) item_max[item] = bm;
 }
 
-__global__ void k_dij_count(const uint32_t* keys, const double* vals, uint64_t cap, const double* item_max,
+__global__ void k_dij_count(const uint32_t* keys, const double* vals, const uint64_t* off, const double* item_max,
                             double rel_thr, uint32_t* counts) {
-    const uint32_t item = blockIdx.x;
-    const uint32_t* k   = keys + (uint64_t) item * cap;
-    const double*   v   = vals + (uint64_t) item * cap;
+    const uint32_t  item = blockIdx.x;
+    const uint64_t  cap  = off[item + 1] - off[item];
+    const uint32_t* k    = keys + off[item];
+    const double*   v    = vals + off[item];
     const double    lim = item_max[item] * rel_thr;
     uint32_t        c   = 0;
     for (uint64_t i = threadIdx.x; i < cap; i += blockDim.x)
@@ -223,11 +226,12 @@ __global__ void k_dij_count(const uint32_t* keys, const double* vals, uint64_t c
     if (threadIdx.x == 0) counts[item] = bc;
 }
 
-__global__ void k_dij_write(const uint32_t* keys, const double* vals, uint64_t cap, const double* item_max,
+__global__ void k_dij_write(const uint32_t* keys, const double* vals, const uint64_t* off, const double* item_max,
                             double rel_thr, const int64_t* row_start, int32_t* cols, float* out_vals) {
-    const uint32_t item = blockIdx.x;
-    const uint32_t* k   = keys + (uint64_t) item * cap;
-    const double*   v   = vals + (uint64_t) item * cap;
+    const uint32_t  item = blockIdx.x;
+    const uint64_t  cap  = off[item + 1] - off[item];
+    const uint32_t* k    = keys + off[item];
+    const double*   v    = vals + off[item];
     const double    lim = item_max[item] * rel_thr;
     int64_t         base = row_start[item];
     typedef hipcub::BlockScan<uint32_t, 256> BS;
@@ -250,17 +254,17 @@ __global__ void k_dij_write(const uint32_t* keys, const double* vals, uint64_t c
 }
 
 int
-dij_compact(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+dij_compact(const uint32_t* keys, const double* vals, const uint64_t* off, uint32_t n_items, double rel_thr,
             double* item_max, uint32_t* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_dij_max, dim3(n_items), dim3(256), 0, s, keys, vals, cap, item_max);
-    hipLaunchKernelGGL(k_dij_count, dim3(n_items), dim3(256), 0, s, keys, vals, cap, item_max, rel_thr, counts);
+    hipLaunchKernelGGL(k_dij_max, dim3(n_items), dim3(256), 0, s, keys, vals, off, item_max);
+    hipLaunchKernelGGL(k_dij_count, dim3(n_items), dim3(256), 0, s, keys, vals, off, item_max, rel_thr, counts);
     return (int) hipGetLastError();
 }
 
 int
-dij_write(const uint32_t* keys, const double* vals, uint64_t cap, uint32_t n_items, double rel_thr,
+dij_write(const uint32_t* keys, const double* vals, const uint64_t* off, uint32_t n_items, double rel_thr,
           const double* item_max, const int64_t* row_start, int32_t* cols, float* out_vals, hipStream_t s) {
-    hipLaunchKernelGGL(k_dij_write, dim3(n_items), dim3(256), 0, s, keys, vals, cap, item_max, rel_thr, row_start,
+    hipLaunchKernelGGL(k_dij_write, dim3(n_items), dim3(256), 0, s, keys, vals, off, item_max, rel_thr, row_start,
                        cols, out_vals);
     return (int) hipGetLastError();
 }

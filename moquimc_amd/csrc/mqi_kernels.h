@@ -78,7 +78,7 @@ struct transport_params {
     int      loops;
     uint32_t* dij_keys;
     double*   dij_vals;
-    uint64_t  dij_capacity;   // slots per item
+    const uint64_t* dij_off;  // item i owns slots [dij_off[i], dij_off[i + 1]) of dij_keys/dij_vals
     // secondary stack
     float* stack;
     int    stack_cap;

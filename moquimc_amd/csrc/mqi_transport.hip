@@ -410,15 +410,20 @@ MQI_DI double letd_hit(const trk& t, float density, bool numer) {
 }
 
 // Dij: per-spot open-addressing table of (voxel, dose) in HBM.  Keys are
-// claimed with one 32-bit CAS, values added with FP64 atomics.
+// claimed with one 32-bit CAS, values added with FP64 atomics.  Linear
+// probing stops after MQI_DIJ_MAX_PROBE slots (a table that full is an
+// overflow: the run reports it instead of crawling through a full table).
+#define MQI_DIJ_MAX_PROBE 4096
 MQI_DI void dij_insert(const transport_params& P, uint64_t item, uint32_t voxel, double value) {
     if (value <= 0) return;
-    const uint64_t cap  = P.dij_capacity;
-    uint32_t*      keys = P.dij_keys + (uint64_t) item * cap;
-    double*        vals = P.dij_vals + (uint64_t) item * cap;
+    const uint64_t base = P.dij_off[item];
+    const uint64_t cap  = P.dij_off[item + 1] - base;
+    uint32_t*      keys = P.dij_keys + base;
+    double*        vals = P.dij_vals + base;
     uint32_t       h    = voxel * 2654435761u;
     uint64_t       slot = ((uint64_t) h * cap) >> 32;
-    for (uint64_t probe = 0; probe < cap; ++probe) {
+    const uint64_t n_probe = cap < MQI_DIJ_MAX_PROBE ? cap : MQI_DIJ_MAX_PROBE;
+    for (uint64_t probe = 0; probe < n_probe; ++probe) {
         uint32_t prev = atomicCAS(keys + slot, 0xffffffffu, voxel);
         if (prev == 0xffffffffu || prev == voxel) {
             atomicAdd(vals + slot, value);

@@ -14,17 +14,22 @@ torch's current stream; ``dose`` wraps them in a ``torch.autograd.Function``.
     f.backward()                      # w.grad = 2 D^T (D w - p) / n
 
 Robust optimisation: ``worst_case_objective`` over the Dij of several
-scenarios (max over scenarios, subgradient of the worst one).
+scenarios (max over scenarios, subgradient of the worst one).  A scenario
+run with a history budget factor b holds b times the dose per unit weight
+(the tables sum over histories, as the reference's unit-weight Dij does);
+``DijMatrix(ctx, s, scale=1 / b)`` puts every scenario on one scale.
 """
 import torch
 
 
 class DijMatrix:
-    """Handle on the device CSR of one scenario of a context's Dij."""
+    """Handle on the device CSR of one scenario of a context's Dij; the
+    products use scale * D."""
 
-    def __init__(self, ctx, scenario=0):
+    def __init__(self, ctx, scenario=0, scale=1.0):
         self.ctx = ctx
         self.scenario = int(scenario)
+        self.scale = float(scale)
         self.nnz, self.n_spots, self.n_voxels = ctx.dij_info(self.scenario)
 
     def _vec(self, x, n):
@@ -38,15 +43,15 @@ class DijMatrix:
         y = torch.empty(n_out, dtype=torch.float64, device=x.device)
         stream = torch.cuda.current_stream(x.device).cuda_stream
         self.ctx.dij_matvec(self.scenario, int(transpose), x.data_ptr(), y.data_ptr(), y_double=True,
-                            stream=stream)
+                            alpha=self.scale, stream=stream)
         return y if out_dtype == torch.float64 else y.to(out_dtype)
 
     def matvec(self, w, out_dtype=torch.float64):
-        """Dose D w (n_voxels)."""
+        """Dose scale * D w (n_voxels)."""
         return self._product(False, self._vec(w, self.n_spots), self.n_voxels, out_dtype)
 
     def rmatvec(self, g, out_dtype=torch.float64):
-        """D^T g (n_spots)."""
+        """scale * D^T g (n_spots)."""
         return self._product(True, self._vec(g, self.n_voxels), self.n_spots, out_dtype)
 
 

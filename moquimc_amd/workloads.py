@@ -15,7 +15,8 @@
   dij     the same CT generator on 256x256x128 at 1.96 x 1.96 x 4 mm (air
           beyond |z| = 200 mm), 5000 spots (25 layers x 200) of 2e4 histories.
   robust  the ct plan under 9 scenarios: nominal, +-3 mm shifts in x, y, z,
-          density x (1 +- 0.035), with nominal weighted 2x.
+          density x (1 +- 0.035); the nominal scenario gets twice the
+          histories of the others.
 
 Positions are in mm, densities in g/mm^3 (the transport's unit).
 """
@@ -64,7 +65,7 @@ class Workload:
         self.spots = np.asarray(spots, _abi.SPOT_DTYPE)
         self.mask = mask
         self.debug = debug
-        self.scenarios = scenarios   # (shifts[n,3], scales[n], weights[n]) or None
+        self.scenarios = scenarios   # (shifts[n,3], scales[n], history budgets[n]) or None
 
     @property
     def shape(self):
@@ -80,7 +81,7 @@ class Workload:
         else:
             ctx.set_grid(self.xe, self.ye, self.ze, self.density)
         if self.scenarios is not None:
-            ctx.set_scenarios(self.scenarios[0], self.scenarios[1])
+            ctx.set_scenarios(*self.scenarios)
         ctx.set_spots(self.spots)
         ctx.set_scorers(self.mask, per_scenario)
 
@@ -214,8 +215,18 @@ def robust_scenarios():
         shifts[1 + k, axis] = 3.0 * sgn
     scales[7] = 1.035
     scales[8] = 0.965
-    weights = np.array([2.0] + [1.0] * 8)
-    return shifts, scales, weights
+    budgets = np.array([2.0] + [1.0] * 8, np.float32)
+    return shifts, scales, budgets
+
+
+def robust(seed=3):
+    """The ct plan (2000 spots, log-uniform histories, 512x512x200 CT) under
+    the 9 robust scenarios in one launch, with the Dij scorer."""
+    w = ct(seed)
+    w.name = ("CT 512x512x200 @0.98x0.98x2 mm, 2000-spot plan (histories log-U[1e3,1e5]) x 9 robust scenarios "
+              "(nominal 2x histories), Dij")
+    w.scenarios = robust_scenarios()
+    return w
 
 
 def target_prescription(xe, ye, ze, radius=30.0, centre=(0.0, 0.0, 0.0), dose=2.0):

@@ -58,7 +58,8 @@ int main(void) {
     F(mqi_spot, mean) F(mqi_spot, sigma) F(mqi_spot, corr) F(mqi_spot, rotation) F(mqi_spot, translation)
     F(mqi_spot, histories)
     F(mqi_run_config, stat_threshold) F(mqi_run_config, max_rounds) F(mqi_run_config, dij)
-    F(mqi_run_config, dij_capacity) F(mqi_run_config, hist_offset)
+    F(mqi_run_config, dij_capacity) F(mqi_run_config, hist_offset) F(mqi_run_config, dij_slots_sqrt)
+    F(mqi_run_config, use_graph)
     F(mqi_run_result, dropped) F(mqi_run_result, steps) F(mqi_run_result, status) F(mqi_run_result, kernel_ms)
     F(mqi_run_result, stat)
     return 0;
@@ -82,7 +83,7 @@ def test_ctypes_mirrors_match_c_layout():
         assert lay["mqi_spot." + f] == _abi.SPOT_DTYPE.fields[f][1], f
     assert lay["mqi_run_config"] == ctypes.sizeof(_abi.RunConfig)
     assert lay["mqi_run_result"] == ctypes.sizeof(_abi.RunResult)
-    for f in ("stat_threshold", "max_rounds", "dij", "dij_capacity", "hist_offset"):
+    for f in ("stat_threshold", "max_rounds", "dij", "dij_capacity", "hist_offset", "dij_slots_sqrt", "use_graph"):
         assert lay["mqi_run_config." + f] == getattr(_abi.RunConfig, f).offset, f
     for f in ("dropped", "steps", "status", "kernel_ms", "stat"):
         assert lay["mqi_run_result." + f] == getattr(_abi.RunResult, f).offset, f

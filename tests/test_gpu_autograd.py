@@ -84,8 +84,10 @@ def test_worst_case_over_scenarios(ctx):
     import torch
     from moquimc_amd import dij
     g, sp = _plan(ctx, scenarios=True)
-    mats = [dij.DijMatrix(ctx, s) for s in range(3)]
-    Ds = [_scipy(ctx, s) for s in range(3)]
+    # per-scenario scales (budget normalisation): products use scale * D
+    sc = [0.5, 1.0, 1.25]
+    mats = [dij.DijMatrix(ctx, s, scale=sc[s]) for s in range(3)]
+    Ds = [_scipy(ctx, s) * sc[s] for s in range(3)]
     # the scenarios see different geometry: their matrices differ
     assert (Ds[0] - Ds[1]).count_nonzero() > 0
     rng = np.random.default_rng(4)

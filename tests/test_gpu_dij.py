@@ -142,3 +142,46 @@ def test_dij_products_long_rows_and_accumulate(ctx):
     assert np.allclose(dose32.cpu().numpy(), want_dose, rtol=2e-6, atol=1e-6 * np.abs(want_dose).max())
     # the empty spot row
     assert D[1].nnz == 0
+
+
+def test_scenario_budgets_and_per_scenario_dij(ctx):
+    """Per-scenario history budgets: the nominal scenario gets twice the
+    histories; each scenario's Dij rows sum to that scenario's dose."""
+    g = phantoms.slab_box(24, 24, 120)
+    sp = phantoms.spots([90.0, 110.0], 400, sigma=3.0, sigma_e_frac=0.01, positions=[(0, 0, 0.5), (4, -2, 0.5)])
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_scenarios(np.array([[0, 0, 0], [1.5, 0, 0], [0, 0, 0]], np.float32),
+                      np.array([1.0, 1.0, 1.05], np.float32), np.array([2.0, 1.0, 0.5], np.float32))
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE, per_scenario=True)
+    res = ctx.run(ctx.config(seed=14, batched=True, dij=True, dij_capacity=1 << 15))
+    assert res.histories == 2 * (800 + 400 + 200)
+    ctx.dij_build(0.0)
+    for s in range(3):
+        D = _csr(ctx, s)
+        col = np.asarray(D.sum(axis=0)).ravel()
+        dose = ctx.scorer(0, scenario=s)
+        assert np.allclose(col, dose, rtol=2e-6, atol=1e-30), s
+    with pytest.raises(Exception):
+        ctx.set_scenarios(np.zeros((2, 3)), np.ones(2), np.array([1.0, -1.0]))
+
+
+def test_sqrt_sized_tables_match_fixed_tables(ctx):
+    """Tables sized dij_slots_sqrt * sqrt(histories) per (scenario, spot) give
+    the same matrix as one fixed capacity (verification kernels, same
+    streams); a table too small for its spot is reported as an overflow."""
+    g, sp = _setup(ctx)
+    sp["histories"] = [40, 150, 0, 600, 150, 90]
+    ctx.set_spots(sp)
+    mats = []
+    for kw in (dict(dij_capacity=1 << 15), dict(dij_slots_sqrt=1024)):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=12, batched=True, exact=True, dij=True, **kw))
+        assert res.status == 0
+        ctx.dij_build(0.0)
+        mats.append(_csr(ctx))
+    a, b = mats
+    assert np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
+    assert np.allclose(a.data, b.data, rtol=1e-6, atol=0)
+    with pytest.raises(RuntimeError, match="overflowed"):
+        ctx.run(ctx.config(seed=12, batched=True, dij=True, dij_slots_sqrt=1, dij_capacity=64))
