@@ -49,7 +49,8 @@ def build(verbose=True):
             (os.path.join(CSRC, "mqi_transport.hip"), os.path.join(tmp, "transport_exact.o"),
              ["-DMQI_EXACT=1", "-ffp-contract=off"]),
             (os.path.join(CSRC, "mqi_transport.hip"), os.path.join(tmp, "transport_fast.o"),
-             ["-DMQI_EXACT=0", "-ffp-contract=fast-honor-pragmas"]),
+             ["-DMQI_EXACT=0", "-ffp-contract=fast-honor-pragmas",
+              "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
             (os.path.join(CSRC, "mqi_aux.hip"), os.path.join(tmp, "aux.o"), ["-ffp-contract=off"]),
             (os.path.join(CSRC, "mqi_api.cpp"), os.path.join(tmp, "api.o"), ["-x", "hip", "-ffp-contract=off"]),
         ]

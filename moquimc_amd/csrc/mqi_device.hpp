@@ -5,7 +5,8 @@
 // the verification build (MQI_EXACT=1: portable math, no FMA contraction)
 // reproduces the CPU restatement bit for bit, history by history.  The
 // production build (MQI_EXACT=0) runs the same code with the device math
-// library and FMA contraction.
+// library, FMA contraction and hardware float division and square root
+// (outside the geometry).
 //
 // Organisation: one lane owns one track at a time.  Its state lives in
 // registers (struct trk); secondaries go to a per-lane LIFO in HBM with the
@@ -94,8 +95,39 @@ MQI_DI v3 mv(const float* m, v3 r) {
     return mk(m[0] * r.x + m[1] * r.y + m[2] * r.z, m[3] * r.x + m[4] * r.y + m[5] * r.z,
               m[6] * r.x + m[7] * r.y + m[8] * r.z);
 }
-// vec3::normalize (mqi_vec.hpp:205-211)
+// Division and square root where the reference's float results are needed
+// to the bit (geometry, track set-up).  The production build compiles float
+// division and square root to the hardware reciprocal / square root (about
+// 1 ulp); these evaluate in double and round once to float, which gives the
+// correctly rounded float result (double rounding is innocuous for division
+// and square root when the wider format has at least 2 * 24 + 2 significand
+// bits).
+MQI_DI float geo_div(float a, float b) {
+#if MQI_EXACT || !defined(__HIP_DEVICE_COMPILE__)
+    return a / b;
+#else
+    return (float) ((double) a / (double) b);
+#endif
+}
+MQI_DI float geo_sqrt(float a) {
+#if MQI_EXACT || !defined(__HIP_DEVICE_COMPILE__)
+    return MQ_SQRTF(a);
+#else
+    return (float) __builtin_sqrt((double) a);
+#endif
+}
+
+// vec3::normalize (mqi_vec.hpp:205-211), correctly rounded in every build
 MQI_DI void normalize(v3& a) {
+    MQI_NO_CONTRACT
+    float n = geo_sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    a.x     = geo_div(a.x, n);
+    a.y     = geo_div(a.y, n);
+    a.z     = geo_div(a.z, n);
+}
+// the same in the physics (scattering rotations): hardware division and
+// square root in the production build
+MQI_DI void normalize_fast(v3& a) {
     MQI_NO_CONTRACT
     float n = MQ_SQRTF(a.x * a.x + a.y * a.y + a.z * a.z);
     a.x /= n;
@@ -264,15 +296,15 @@ MQI_DI void axis_in(float& dcomp, float p, float v1, float v2, int i, int n, flo
     float       me  = dcomp;
     if (me * me > 1e-7f) {
         if (me < 0) {
-            if (fabsf(-(p - v1) / dcomp) < tol && i > 0)
+            if (fabsf(geo_div(-(p - v1), dcomp)) < tol && i > 0)
                 tmax = 1 / tol;
             else
-                tmax = -(p - v1) / dcomp;
+                tmax = geo_div(-(p - v1), dcomp);
         } else {
-            if (fabsf((v2 - p) / dcomp) < tol && i < n)
+            if (fabsf(geo_div(v2 - p, dcomp)) < tol && i < n)
                 tmax = 1 / tol;
             else
-                tmax = (v2 - p) / dcomp;
+                tmax = geo_div(v2 - p, dcomp);
         }
     } else {
         dcomp = 0;
@@ -302,11 +334,11 @@ MQI_DI void axis_out(float& dcomp, float p, float lo, float hi, float& tmin, flo
     float me = dcomp;
     if (me * me > 1e-7f) {
         if (me > 0) {
-            tmin = (lo - p) / dcomp;
-            tmax = (hi - p) / dcomp;
+            tmin = geo_div(lo - p, dcomp);
+            tmax = geo_div(hi - p, dcomp);
         } else {
-            tmax = (lo - p) / dcomp;
-            tmin = (hi - p) / dcomp;
+            tmax = geo_div(lo - p, dcomp);
+            tmin = geo_div(hi - p, dcomp);
         }
     } else {
         dcomp = 0;
@@ -413,7 +445,7 @@ MQI_DI void rotate_direction(v3& d, float theta, float phi) {
     if (phi != 0) rotate_z(m, phi);
     const v3 ref = mk(0, 0, 1);
     v3       dl  = mv(m, ref);
-    normalize(dl);
+    normalize_fast(dl);
     // mat3x3(f = ref, t = d)
     v3    v  = mk(ref.y * d.z - ref.z * d.y, ref.z * d.x - ref.x * d.z, ref.x * d.y - ref.y * d.x);
     float fn = MQ_SQRTF(ref.x * ref.x + ref.y * ref.y + ref.z * ref.z);
@@ -423,8 +455,8 @@ MQI_DI void rotate_direction(v3& d, float theta, float phi) {
     if (fabsf(c - 1) < 1e-3f || fabsf(c + 1) < 1e-3f) {
         v3 uu = mk(1 - ref.x, 0 - ref.y, 0 - ref.z);
         v3 vv = mk(1 - d.x, 0 - d.y, 0 - d.z);
-        normalize(uu);
-        normalize(vv);
+        normalize_fast(uu);
+        normalize_fast(vv);
         float dot_u  = uu.x * uu.x + uu.y * uu.y + uu.z * uu.z;
         float dot_v  = vv.x * vv.x + vv.y * vv.y + vv.z * vv.z;
         float dot_uv = vv.x * uu.x + vv.y * uu.y + vv.z * uu.z;
@@ -447,7 +479,7 @@ MQI_DI void rotate_direction(v3& d, float theta, float phi) {
         g[8]    = c + h * v.z * v.z;
     }
     d = mv(g, dl);
-    normalize(d);
+    normalize_fast(d);
 }
 
 // ------------------------------------------------------------------ random numbers

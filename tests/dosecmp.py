@@ -65,11 +65,15 @@ def gamma_pass_rate(ref, ev, spacing, dd=0.01, dta=1.0, threshold=0.1, refine=1)
 
 def r80(depth_centres, idd):
     """Distal depth where the integral depth dose falls to 80 % of its
-    maximum (linear interpolation between voxel centres)."""
+    maximum for the last time (linear interpolation between voxel centres).
+    For a single Bragg peak this is the one crossing beyond the maximum; a
+    field of several energies through heterogeneous slabs can touch the 80 %
+    level between its peaks, where the first crossing would flip on noise,
+    while the distal falloff is steep and well defined."""
     idd = np.asarray(idd, np.float64)
     k = int(np.argmax(idd))
     level = 0.8 * idd[k]
-    for i in range(k, len(idd) - 1):
+    for i in range(len(idd) - 2, k - 1, -1):
         if idd[i] >= level > idd[i + 1]:
             t = (idd[i] - level) / (idd[i] - idd[i + 1])
             return float(depth_centres[i] + t * (depth_centres[i + 1] - depth_centres[i]))

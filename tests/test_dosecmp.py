@@ -71,3 +71,11 @@ def test_two_sigma_expectation_matches_student_t():
     want = dosecmp.two_sigma_expectation(sa, sb, nb - 1, nb - 1, mask)
     assert 0.90 < want < 0.93
     assert abs(frac - want) < 0.01
+
+
+def test_r80_takes_the_distal_falloff_of_a_multi_peak_curve():
+    # a dip to 75 % of the maximum between two peaks, then a linear distal
+    # falloff from 100 % at z = 60 to 0 at z = 70: R80 = 62, not the dip
+    z = np.arange(0.0, 80.0, 1.0)
+    idd = np.where(z < 20, 1.0, np.where(z < 40, 0.75, np.where(z <= 60, 1.0, np.clip(1.0 - (z - 60) / 10.0, 0, 1))))
+    assert abs(dosecmp.r80(z, idd) - 62.0) < 1e-12
