@@ -37,10 +37,13 @@
 #define MQ_SINF(x) mqi_pm::pm_sinf(x)
 #define MQ_ACOSF(x) mqi_pm::pm_acosf(x)
 #else
+// production: device library, except pow = exp2(y log2 x) on the hardware
+// log2 / exp2 (about 2 ulp here; the library's powf costs tens of
+// instructions and stopping_power_ratio calls it twice per step)
 #define MQ_LOGF(x) logf(x)
 #define MQ_LOGD(x) log(x)
 #define MQ_EXPD(x) exp(x)
-#define MQ_POWF(x, y) powf(x, y)
+#define MQ_POWF(x, y) __builtin_amdgcn_exp2f((y) * __builtin_amdgcn_logf(x))
 #define MQ_SQRTF(x) sqrtf(x)
 #define MQ_SQRTD(x) sqrt(x)
 #define MQ_COSF(x) cosf(x)
@@ -150,9 +153,16 @@ MQI_DI rel_q relq(float Ek) {
     const float   MeMp_sq = MeMp * MeMp;
     r.gamma               = r.Et / Mp;
     r.gamma_sq            = r.gamma * r.gamma;
-    r.beta_sq             = 1.0 - 1.0 / r.gamma_sq;
-    r.Te_max              = (2.0 * Me * r.beta_sq * r.gamma_sq);
+#if MQI_EXACT
+    r.beta_sq = 1.0 - 1.0 / r.gamma_sq;
+    r.Te_max  = (2.0 * Me * r.beta_sq * r.gamma_sq);
     r.Te_max /= (1.0 + 2.0 * r.gamma * MeMp + MeMp_sq);
+#else
+    // production: float, and beta^2 = Ek (Ek + 2 Mp) / Et^2, the same
+    // quantity without the cancellation of 1 - 1 / gamma^2 at low energy
+    r.beta_sq = Ek * (Ek + 2.0f * Mp) / r.Et_sq;
+    r.Te_max  = (2.0f * Me * r.beta_sq * r.gamma_sq) / (1.0f + 2.0f * r.gamma * MeMp + MeMp_sq);
+#endif
     return r;
 }
 MQI_DI float momentum(const rel_q& r) {
@@ -193,9 +203,17 @@ MQI_DI float radiation_length(float rho_mass) {
     } else if (density > 0.26 && density <= 0.9) {
         f = 1.0446 - 0.2180 * density;
     } else if (density > 0.9) {
+#if MQI_EXACT
         f = 1.19 + 0.44 * MQ_LOGD((double) (density - 0.44));
+#else
+        f = 1.19f + 0.44f * __logf(density - 0.44f);   // production: float
+#endif
     }
+#if MQI_EXACT
     return (float) ((units::water_density * units::X0_water) / (density * 0.001 * f));
+#else
+    return (units::water_density * units::X0_water) / (density * 0.001f * f);
+#endif
 }
 
 // hu_to_density (mqi_patient_materials.hpp:515-542)

@@ -395,7 +395,12 @@ MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b) {
 template<bool DBG>
 MQI_DI double dose_hit(const trk& t, float density, float volume) {
     if (density < 1.0e-7) return 0.0;
+#if MQI_EXACT
     return (t.dE + t.ldE) * 1.60218e-10 / (volume * density * spr<DBG>(density, t.ke0));
+#else
+    // production: the per-step value in float (it is accumulated in FP64)
+    return (double) ((t.dE + t.ldE) * 1.60218e-10f / (volume * density * spr<DBG>(density, t.ke0)));
+#endif
 }
 MQI_DI double letd_hit(const trk& t, float density, bool numer) {
     density *= 1000.0;
