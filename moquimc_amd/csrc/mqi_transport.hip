@@ -90,9 +90,9 @@ MQI_DI float dedx(const ctx& c, float Ek) {
 // residual-range walk stops at index 0 where the reference's uint16_t
 // counter would wrap past zero.
 template<bool DBG>
-MQI_DI float energy_loss(ctx& c, const rel_q& rel, float rho, float step_length) {
+MQI_DI float energy_loss(ctx& c, const rel_q& rel, float rho, float step_length, float sp) {
     const float* R   = c.T + T_RANGE;
-    float        L_w = step_length * spr<DBG>(rho, rel.Ek) * rho / units::water_density;
+    float        L_w = step_length * sp * rho / units::water_density;
     uint16_t     n   = uint16_t((rel.Ek - 0.1f) / 0.5f);
     float        x0  = 0.1f + n * 0.5f;
     float        x1  = x0 + 0.5f;
@@ -120,9 +120,9 @@ MQI_DI float energy_loss(ctx& c, const rel_q& rel, float rho, float step_length)
 
 // along_step (mqi_p_ionization.hpp:348-420)
 template<bool DBG>
-MQI_DI void along_step(ctx& c, trk& t, float len, float rho) {
+MQI_DI void along_step(ctx& c, trk& t, float len, float rho, float sp) {
     rel_q rel = relq(t.ke0);
-    float dE  = energy_loss<DBG>(c, rel, rho, len);
+    float dE  = energy_loss<DBG>(c, rel, rho, len, sp);
     float r   = 1.0;
     if (dE >= t.ke0) {
         r         = t.ke0 / dE;
@@ -320,32 +320,39 @@ MQI_DI void post_poi(ctx& c, trk& t) {
 
 // fippel_physics::stepping (mqi_fippel_physics.hpp:68-216).  The three
 // branches' along_step calls are merged into one call with the branch's
-// length; the random-number order is unchanged.
+// length; the random-number order is unchanged.  The stopping power ratio
+// the reference evaluates at several points of a step (stepping,
+// energy_loss, last_step, the dose hit) always has the arguments (rho,
+// ke0 of the step): it is evaluated once and returned in sp for scoring.
 template<bool DBG>
-MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b) {
+MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b, float& sp) {
+    sp = 0.0f;   // vacuum: the dose hit returns before reading it
     if (rho < 1.0e-7) {
         update_pos(t, d2b);
         return;
     } else if (rho > 99.9) {
         t.stopped = true;
+        sp        = spr<DBG>(rho, t.ke0);
         return;
     }
     if (t.ke0 <= 0.5f) {
         if (t.ke0 < 0) t.ke0 = 0;
+        sp = spr<DBG>(rho, t.ke0);
         t.dE += t.ke0;
         t.ke1 -= t.ke0;
         // last_step (mqi_p_ionization.hpp:481-490)
         rel_q rel = relq(t.ke0);
         float L_w = 0;
         if (t.dE > 0) L_w = -t.dE / dedx(c, rel.Ek);
-        float step_length = L_w * units::water_density / (spr<DBG>(rho, t.ke0) * rho);
+        float step_length = L_w * units::water_density / (sp * rho);
         update_pos(t, step_length);
         t.stopped = true;
         return;
     }
     rel_q rel = relq(t.ke0);
     float cms = 1.0f;
-    cms       = cms * spr<DBG>(rho, rel.Ek) * rho / units::water_density;
+    sp        = spr<DBG>(rho, rel.Ek);   // rel.Ek == t.ke0
+    cms       = cms * sp * rho / units::water_density;
     float max_loss_energy = -1.0 * cms * dedx(c, rel.Ek);
     float cs1[4]  = { cs_ion(c, rel.Ek, rho), cs_tab(c, T_CS_PP, rel.Ek, rho), cs_tab(c, T_CS_POE, rel.Ek, rho),
                       cs_tab(c, T_CS_POI, rel.Ek, rho) };
@@ -360,7 +367,7 @@ MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b) {
           csc = first ? cs1[3] : cs2[3];
     float prob       = c.rng.uniform();
     float mfp        = -1.0f * MQ_LOGF(prob) / cs_sum;
-    float step_limit = cms * units::water_density / (spr<DBG>(rho, rel.Ek) * rho);
+    float step_limit = cms * units::water_density / (sp * rho);
     int   br;
     float len;
     if (d2b < mfp && d2b < step_limit) {
@@ -374,7 +381,7 @@ MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b) {
         br  = 2;
         len = step_limit;
     }
-    along_step<DBG>(c, t, len, rho);
+    along_step<DBG>(c, t, len, rho, sp);
     if (br != 1) return;
     if (t.ke1 <= 0.5f) return;
     float u = cs_sum * c.rng.uniform();
@@ -393,13 +400,13 @@ MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b) {
 // ------------------------------------------------------------------ scoring
 // hit functions (mqi_scorer_energy_deposit.hpp:43-139)
 template<bool DBG>
-MQI_DI double dose_hit(const trk& t, float density, float volume) {
+MQI_DI double dose_hit(const trk& t, float density, float volume, float sp) {
     if (density < 1.0e-7) return 0.0;
 #if MQI_EXACT
-    return (t.dE + t.ldE) * 1.60218e-10 / (volume * density * spr<DBG>(density, t.ke0));
+    return (t.dE + t.ldE) * 1.60218e-10 / (volume * density * sp);
 #else
     // production: the per-step value in float (it is accumulated in FP64)
-    return (double) ((t.dE + t.ldE) * 1.60218e-10f / (volume * density * spr<DBG>(density, t.ke0)));
+    return (double) ((t.dE + t.ldE) * 1.60218e-10f / (volume * density * sp));
 #endif
 }
 MQI_DI double letd_hit(const trk& t, float density, bool numer) {
@@ -463,14 +470,15 @@ MQI_DI void flush(const transport_params& P, pending& A) {
 }
 
 template<bool DBG>
-MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float density, float volume,
+MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float density, float volume, float sp,
                   uint64_t dij_item, uint64_t scen_off, pending& A) {
     const int     n   = P.n_scorers;
     const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT
     // Both loops run over the fixed maximum with constant indices so that the
     // scorer table stays in kernel arguments / registers (a runtime index
     // into it would move the whole argument block to private memory).
-    double v[MQI_MAX_SCORERS];
+    const double d = dose_hit<DBG>(t, density, volume, sp);
+    double       v[MQI_MAX_SCORERS];
 #pragma unroll
     for (int s = 0; s < MQI_MAX_SCORERS; ++s) {
         v[s] = 0.0;
@@ -480,10 +488,8 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             v[s] = letd_hit(t, density, true);
         else if (k == MQI_KIND_LETD_DEN)
             v[s] = letd_hit(t, density, false);
-        else {
-            double d = dose_hit<DBG>(t, density, volume);
-            v[s]     = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
-        }
+        else
+            v[s] = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
     }
     if (P.loops == MQI_LOOPS_ONCE) {
         if ((int64_t) cnb != A.cnb || scen_off != A.off) {
@@ -681,7 +687,8 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L) {
     const uint64_t cnb = (uint64_t) (int32_t) (t.cz * c.g.nx * c.g.ny + t.cy * c.g.nx + t.cx);
     t.its              = intersect_in(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
     const float rho    = P.density[cnb] * L.scale;
-    stepping<DBG>(c, t, rho, t.its);
+    float sp;   // stopping power ratio of the step
+    stepping<DBG>(c, t, rho, t.its, sp);
     if (t.its < 0) {
         L.active = false;
         return;
@@ -695,7 +702,7 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L) {
         float     vol = c.g.X(i + 1) - c.g.X(i);
         vol *= c.g.Y(j + 1) - c.g.Y(j);
         vol *= c.g.Z(k + 1) - c.g.Z(k);
-        score<DBG>(P, t, cnb, rho, vol, L.dij_item, L.scen_off, L.acc);
+        score<DBG>(P, t, cnb, rho, vol, sp, L.dij_item, L.scen_off, L.acc);
     }
     if (!t.stopped) {
         step_axis(t.p1.x, t.d1.x, c.g.xe, c.g.sx, t.cx);
