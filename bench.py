@@ -28,6 +28,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+TRAFFIC_SOURCE = "profiles/hbm_counters_ct2000_dij5000.md (rocprofv3 FETCH_SIZE and WRITE_SIZE, copy-calibrated)"
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of a kernel from the committed counter runs, or
+    None when they are absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)[key]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def transport_roofline(steps, kernel_ms, launches, n_scorers):
     """Algorithmic bytes of the transport: per voxel step one density load
     (4 B) and one FP64 read-modify-write per scorer (8 B each), over the
@@ -36,11 +50,15 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers):
     per_launch = steps / launches * bytes_per_step
     ms_per_launch = kernel_ms / launches
     achieved = per_launch / (ms_per_launch * 1e-3) / 1e9
+    m = measured_traffic("k_transport")
+    # counters per voxel step, scaled to this launch's steps
+    traffic = None if m is None else round(m["traffic_bytes"] / m["voxel_steps"] * steps / launches)
     return {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+            "traffic_source": TRAFFIC_SOURCE if m else None,
             "kernel": "k_transport<release, dynamic> (persistent)", "launch_ms": round(ms_per_launch, 3),
             "voxel_steps_per_launch": int(steps / launches), "bytes_per_step": bytes_per_step,
-            "atomics_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1)}
+            "scorer_deposits_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1)}
 
 
 def cpu_baseline(w, cores, target_s=20.0):
@@ -131,7 +149,10 @@ def adjoint_bench(ctx_cls, reps=20):
             "adjoint_ms": round(ms_med, 4), "adjoint_GBs": round(gbs, 1),
             "forward_ms": round(f_med, 4), "forward_GBs": round(f_bytes / (f_med * 1e-3) / 1e9, 1),
             "adjoint_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                 "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                                 "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                 "traffic": (round(measured_traffic("k_csr_adaptive_adjoint")["traffic_bytes"])
+                                             if measured_traffic("k_csr_adaptive_adjoint") else None),
+                                 "traffic_source": TRAFFIC_SOURCE,
                                  "kernel": "k_csr_adaptive + k_csr_split_rows (D^T g)", "algorithmic_bytes": int(alg_bytes)}}
 
 
