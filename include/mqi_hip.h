@@ -38,7 +38,8 @@ enum {
     MQI_E_STATE      = 2,   /* call order: grid / spots / scorers missing */
     MQI_E_HIP        = 3,   /* HIP runtime error */
     MQI_E_MEMORY     = 4,   /* device allocation failed */
-    MQI_E_CAPACITY   = 5    /* a fixed-size table (Dij hash) overflowed */
+    MQI_E_CAPACITY   = 5,   /* a fixed-size table (Dij hash) overflowed */
+    MQI_E_IO         = 6    /* an output file could not be written */
 };
 
 /* scoring-loop semantics of a run */
@@ -192,6 +193,27 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
  * run two of them concurrently on different streams. */
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
+
+/* Writes the Dij of a scenario as the reference's scipy-style CSR .npz
+ * (io::save_to_npz, mqi_io.hpp:249-320, zip writer mqi_sparse_io.hpp:166-400):
+ * an uncompressed zip of indices.npy and indptr.npy (uint32), shape.npy
+ * (uint32 [spots, voxels]), data.npy (float64, value * scale) and format.npy
+ * ("csr").  scale: the reference's particles_per_history * rbe * n_fractions.
+ * Rows hold mqi_dij_build's entries (columns sorted, threshold applied). */
+int mqi_dij_save_npz(mqi_ctx* ctx, int scenario, const char* path, double scale);
+
+/* output formats of mqi_save_scorer */
+enum {
+    MQI_FORMAT_RAW = 0,   /* <dir>/<name>.raw, float64 (io::save_to_bin, mqi_io.hpp:165-183) */
+    MQI_FORMAT_MHD = 1,   /* <dir>/<name>.mhd + .raw (io::save_to_mhd, mqi_io.hpp:493-537) */
+    MQI_FORMAT_MHA = 2    /* <dir>/<name>.mha, data after the header (io::save_to_mha, mqi_io.hpp:539-591) */
+};
+
+/* Writes scorer k of a scenario, times scale, in one of the reference's dense
+ * formats (the save_reshaped_files loop, mqi_tps_env.hpp:1840-1895); the
+ * MetaImage header carries the grid's spacing and first voxel centre. */
+int mqi_save_scorer(mqi_ctx* ctx, int k, int scenario, const char* dir, const char* name, int format,
+                    double scale);
 
 /* Device time (ms, HIP events) of the last mqi_dij_matvec, for measurement;
  * waits for that product to finish. */

@@ -86,6 +86,8 @@ def lib():
             "mqi_last_matvec_ms": (i, [vp, ctypes.POINTER(d)]),
             "mqi_autotune": (i, [vp, ctypes.POINTER(RunConfig), d, ctypes.POINTER(d)]),
             "mqi_math_probe": (i, [vp, i, vp, vp, ctypes.c_int64, vp]),
+            "mqi_dij_save_npz": (i, [vp, i, ctypes.c_char_p, d]),
+            "mqi_save_scorer": (i, [vp, i, i, ctypes.c_char_p, ctypes.c_char_p, i, d]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -99,7 +101,10 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_get_density", "mqi_set_spots", "mqi_set_scenarios", "mqi_set_scorers", "mqi_zero_scorers",
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
-            "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune"]
+            "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
+            "mqi_dij_save_npz", "mqi_save_scorer"]
+
+FORMATS = {"raw": 0, "mhd": 1, "mha": 2}
 
 
 class MqiError(RuntimeError):
@@ -277,6 +282,16 @@ class Context:
         val = np.zeros(max(nnz, 1), np.float32)
         self._chk(self._L.mqi_dij_copy(self.h, scenario, _p(indptr), _p(idx), _p(val)), "mqi_dij_copy")
         return indptr, idx[:nnz], val[:nnz], (rows, cols)
+
+    def dij_save_npz(self, path, scenario=0, scale=1.0):
+        """The reference's scipy-style CSR .npz of a scenario's Dij."""
+        self._chk(self._L.mqi_dij_save_npz(self.h, scenario, os.fsencode(path), ctypes.c_double(scale)),
+                  "mqi_dij_save_npz")
+
+    def save_scorer(self, k, directory, name, fmt="raw", scenario=0, scale=1.0):
+        """Scorer k as <directory>/<name>.raw, .mhd + .raw or .mha (float64)."""
+        self._chk(self._L.mqi_save_scorer(self.h, k, scenario, os.fsencode(directory), os.fsencode(name),
+                                          FORMATS[fmt], ctypes.c_double(scale)), "mqi_save_scorer")
 
     def dij_matvec(self, scenario, transpose, x_ptr, y_ptr, y_double=True, alpha=1.0, accumulate=False, stream=None):
         self._chk(self._L.mqi_dij_matvec(self.h, scenario, int(transpose), ctypes.c_void_p(x_ptr),

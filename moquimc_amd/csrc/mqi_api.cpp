@@ -19,6 +19,7 @@
 #include "../../include/mqi_hip.h"
 #include "mqi_aux.h"
 #include "mqi_kernels.h"
+#include "../host/mqi_hip_io.hpp"
 
 namespace
 {
@@ -1313,6 +1314,45 @@ mqi_dij_copy(mqi_ctx* c, int s, int64_t* indptr, int32_t* indices, float* values
     if (indptr) HIPCHK(c, hipMemcpy(indptr, m.indptr, sizeof(int64_t) * (m.rows + 1), hipMemcpyDeviceToHost));
     if (indices && m.nnz) HIPCHK(c, hipMemcpy(indices, m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
     if (values && m.nnz) HIPCHK(c, hipMemcpy(values, m.val, sizeof(float) * m.nnz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int
+mqi_dij_save_npz(mqi_ctx* c, int s, const char* path, double scale) {
+    if (!c || !path) return MQI_E_ARG;
+    if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);
+    const csr_dev& m = c->dij[s];
+    if (m.cols > (int64_t) UINT32_MAX) return fail(c, MQI_E_ARG, "%lld voxels exceed the uint32 format", (long long) m.cols);
+    std::vector<int64_t> ip(m.rows + 1);
+    std::vector<int32_t> ix((size_t) m.nnz);
+    std::vector<float>   v((size_t) m.nnz);
+    int                  r = mqi_dij_copy(c, s, ip.data(), ix.data(), v.data());
+    if (r) return r;
+    try {
+        mqi_hip::io::save_npz_csr(path, (uint32_t) m.rows, (uint32_t) m.cols, ip.data(), ix.data(), v.data(), scale);
+    } catch (const std::exception& e) {
+        return fail(c, MQI_E_IO, "%s", e.what());
+    }
+    return 0;
+}
+
+int
+mqi_save_scorer(mqi_ctx* c, int k, int scenario, const char* dir, const char* name, int format, double scale) {
+    int r = ready(c);
+    if (r) return r;
+    if (!dir || !name) return MQI_E_ARG;
+    if (format < MQI_FORMAT_RAW || format > MQI_FORMAT_MHA) return fail(c, MQI_E_ARG, "unknown format %d", format);
+    std::vector<double> d(nvox(c));
+    if ((r = mqi_get_scorer(c, k, scenario, d.data()))) return r;
+    try {
+        if (format == MQI_FORMAT_RAW)
+            mqi_hip::io::save_to_bin(d.data(), d.size(), scale, dir, name);
+        else
+            mqi_hip::io::save_metaimage(d.data(), c->nx, c->ny, c->nz, c->h_xe.data(), c->h_ye.data(), c->h_ze.data(),
+                                        scale, dir, name, format == MQI_FORMAT_MHA);
+    } catch (const std::exception& e) {
+        return fail(c, MQI_E_IO, "%s", e.what());
+    }
     return 0;
 }
 

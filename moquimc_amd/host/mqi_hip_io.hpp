@@ -1,0 +1,285 @@
+// Output writers in the reference's on-disk formats, so that tools reading
+// moqui's outputs read these unchanged:
+//
+//   Dij as a scipy-style CSR .npz (io::save_to_npz, mqi_io.hpp:249-320, and
+//   the zip writer io::save_npz, mqi_sparse_io.hpp:166-400): an uncompressed
+//   zip of indices.npy (uint32), indptr.npy (uint32), shape.npy (uint32
+//   [rows, cols]), data.npy (float64, value * scale) and format.npy ("csr");
+//   rows = spots, columns = voxels.
+//   dense scorers as raw float64 (io::save_to_bin, mqi_io.hpp:165-183) and
+//   MetaImage .mhd + .raw / .mha (io::save_to_mhd / save_to_mha,
+//   mqi_io.hpp:493-591), with the reference's header text.
+//
+// Differences by design: CSR rows come from mqi_dij_build (columns sorted,
+// the build's threshold applied) where the reference writes its hash-table
+// order; a member above 4 GiB - 1 bytes is refused (the reference's 32-bit
+// zip fields would wrap); errors throw mqi_hip::io_error instead of
+// printing and carrying on.
+#ifndef MQI_HIP_IO_HPP
+#define MQI_HIP_IO_HPP
+
+#include <array>
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mqi_hip
+{
+namespace io
+{
+
+struct io_error : std::runtime_error {
+    explicit io_error(const std::string& w) : std::runtime_error(w) {}
+};
+
+// CRC-32 (ISO-HDLC: reflected polynomial 0xEDB88320, initial and final
+// value 0xFFFFFFFF), the checksum of the zip format, byte-wise with a table
+inline uint32_t
+crc32_update(uint32_t crc, const void* data, size_t n) {
+    static const std::array<uint32_t, 256> table = [] {
+        std::array<uint32_t, 256> t{};
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            t[i] = c;
+        }
+        return t;
+    }();
+    const uint8_t* p = static_cast<const uint8_t*>(data);
+    crc              = ~crc;
+    for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return ~crc;
+}
+
+// .npy version 1.0 header: magic, version, header length, then the dict
+// padded with spaces and a newline to a multiple of 64 bytes
+inline std::string
+npy_header(const std::string& descr, const std::vector<size_t>& shape) {
+    std::ostringstream d;
+    d << "{'descr': '" << descr << "', 'fortran_order': False, 'shape': (";
+    for (size_t i = 0; i < shape.size(); ++i) d << shape[i] << (shape.size() == 1 ? "," : (i + 1 < shape.size() ? ", " : ""));
+    d << "), }";
+    std::string dict = d.str();
+    const size_t base = 10;
+    size_t       len  = dict.size() + 1;
+    size_t       pad  = (64 - (base + len) % 64) % 64;
+    dict.append(pad, ' ');
+    dict.push_back('\n');
+    std::string h("\x93NUMPY", 6);
+    h.push_back('\x01');
+    h.push_back('\x00');
+    const uint16_t hl = (uint16_t) dict.size();
+    h.push_back((char) (hl & 0xFF));
+    h.push_back((char) (hl >> 8));
+    return h + dict;
+}
+
+// an uncompressed zip archive written member by member
+class zip_writer
+{
+public:
+    explicit zip_writer(const std::string& path) : path_(path), f_(path, std::ios::binary | std::ios::trunc) {
+        if (!f_) throw io_error("cannot write " + path);
+    }
+
+    // one .npy member: header + raw little-endian payload
+    void
+    add_npy(const std::string& name, const std::string& descr, const std::vector<size_t>& shape, const void* data,
+            size_t nbytes) {
+        const std::string hdr   = npy_header(descr, shape);
+        const uint64_t    total = (uint64_t) hdr.size() + nbytes;
+        if (total > 0xFFFFFFFEull) throw io_error(name + " in " + path_ + " exceeds the 4 GiB zip member limit");
+        uint32_t crc = crc32_update(0, hdr.data(), hdr.size());
+        crc          = crc32_update(crc, data, nbytes);
+        entry e{name, crc, (uint32_t) total, (uint32_t) offset_};
+        std::string lh;
+        put32(lh, 0x04034b50u);   // local file header
+        put16(lh, 20);            // version needed
+        put16(lh, 0);             // flags
+        put16(lh, 0);             // stored
+        put16(lh, 0);             // time
+        put16(lh, 0);             // date
+        put32(lh, crc);
+        put32(lh, e.size);
+        put32(lh, e.size);
+        put16(lh, (uint16_t) name.size());
+        put16(lh, 0);
+        lh += name;
+        write(lh.data(), lh.size());
+        write(hdr.data(), hdr.size());
+        write(data, nbytes);
+        entries_.push_back(e);
+    }
+
+    void
+    close() {
+        std::string cd;
+        for (const entry& e : entries_) {
+            put32(cd, 0x02014b50u);   // central directory header
+            put16(cd, 20);            // version made by
+            put16(cd, 20);            // version needed
+            put16(cd, 0);
+            put16(cd, 0);
+            put16(cd, 0);
+            put16(cd, 0);
+            put32(cd, e.crc);
+            put32(cd, e.size);
+            put32(cd, e.size);
+            put16(cd, (uint16_t) e.name.size());
+            put16(cd, 0);   // extra
+            put16(cd, 0);   // comment
+            put16(cd, 0);   // disk
+            put16(cd, 0);   // internal attributes
+            put32(cd, 0);   // external attributes
+            put32(cd, e.offset);
+            cd += e.name;
+        }
+        const uint64_t cd_off = offset_;
+        if (cd_off > 0xFFFFFFFEull) throw io_error(path_ + " exceeds the 4 GiB zip limit");
+        std::string end;
+        put32(end, 0x06054b50u);   // end of central directory
+        put16(end, 0);
+        put16(end, 0);
+        put16(end, (uint16_t) entries_.size());
+        put16(end, (uint16_t) entries_.size());
+        put32(end, (uint32_t) cd.size());
+        put32(end, (uint32_t) cd_off);
+        put16(end, 0);
+        write(cd.data(), cd.size());
+        write(end.data(), end.size());
+        f_.close();
+        if (!f_) throw io_error("error writing " + path_);
+    }
+
+private:
+    struct entry {
+        std::string name;
+        uint32_t    crc, size, offset;
+    };
+    static void
+    put16(std::string& s, uint16_t v) {
+        s.push_back((char) (v & 0xFF));
+        s.push_back((char) (v >> 8));
+    }
+    static void
+    put32(std::string& s, uint32_t v) {
+        for (int k = 0; k < 4; ++k) s.push_back((char) ((v >> (8 * k)) & 0xFF));
+    }
+    void
+    write(const void* p, size_t n) {
+        f_.write(static_cast<const char*>(p), (std::streamsize) n);
+        if (!f_) throw io_error("error writing " + path_);
+        offset_ += n;
+    }
+
+    std::string        path_;
+    std::ofstream      f_;
+    uint64_t           offset_ = 0;
+    std::vector<entry> entries_;
+};
+
+// Dij CSR (rows = spots, columns = voxels) as the reference's .npz: data =
+// value * scale in float64, uint32 indices and indptr, uint32 shape
+inline void
+save_npz_csr(const std::string& path, uint32_t rows, uint32_t cols, const int64_t* indptr, const int32_t* indices,
+             const float* values, double scale) {
+    const uint64_t nnz = (uint64_t) indptr[rows];
+    if (nnz > std::numeric_limits<uint32_t>::max()) throw io_error("nnz exceeds the uint32 indptr of the format");
+    std::vector<uint32_t> ip(rows + 1), ix(nnz);
+    std::vector<double>   dv(nnz);
+    for (uint32_t r = 0; r <= rows; ++r) ip[r] = (uint32_t) indptr[r];
+    for (uint64_t i = 0; i < nnz; ++i) {
+        ix[i] = (uint32_t) indices[i];
+        dv[i] = (double) values[i] * scale;
+    }
+    const uint32_t shape[2] = {rows, cols};
+    zip_writer     z(path);
+    z.add_npy("indices.npy", "<u4", {(size_t) nnz}, ix.data(), ix.size() * 4);
+    z.add_npy("indptr.npy", "<u4", {(size_t) rows + 1}, ip.data(), ip.size() * 4);
+    z.add_npy("shape.npy", "<u4", {2}, shape, sizeof shape);
+    z.add_npy("data.npy", "<f8", {(size_t) nnz}, dv.data(), dv.size() * 8);
+    z.add_npy("format.npy", "|S3", {}, "csr", 3);
+    z.close();
+}
+
+// dense scorer * scale as raw float64 (<dir>/<name>.raw)
+inline void
+save_to_bin(const double* src, size_t n, double scale, const std::string& dir, const std::string& name) {
+    std::vector<double> d(src, src + n);
+    for (double& v : d) v *= scale;
+    std::ofstream f(dir + "/" + name + ".raw", std::ios::binary | std::ios::trunc);
+    f.write(reinterpret_cast<const char*>(d.data()), (std::streamsize) (n * sizeof(double)));
+    if (!f) throw io_error("error writing " + dir + "/" + name + ".raw");
+}
+
+// the reference's voxel spacing and first-voxel centre: edge[1] - edge[0]
+// and edge[0] + d * 0.5, rounded to float (uniform grids, as there)
+inline void
+grid_origin(const float* e, float& d, float& o) {
+    d = e[1];
+    d -= e[0];
+    o = (float) (e[0] + d * 0.5);
+}
+
+// MetaImage: <dir>/<name>.mhd header + <name>.raw (mha = false), or one
+// <dir>/<name>.mha with the data after the header (mha = true)
+inline void
+save_metaimage(const double* src, int nx, int ny, int nz, const float* xe, const float* ye, const float* ze,
+               double scale, const std::string& dir, const std::string& name, bool mha) {
+    float dx, dy, dz, x0, y0, z0;
+    grid_origin(xe, dx, x0);
+    grid_origin(ye, dy, y0);
+    grid_origin(ze, dz, z0);
+    const size_t        n = (size_t) nx * ny * nz;
+    std::vector<double> d(src, src + n);
+    for (double& v : d) v *= scale;
+    std::ostringstream h;
+    h << "ObjectType = Image\n";
+    h << "NDims = 3\n";
+    h << "BinaryData = True\n";
+    h << "BinaryDataByteOrderMSB = False\n";
+    h << "CompressedData = False\n";
+    if (!mha) {
+        h << "TransformMatrix 1 0 0 0 1 0 0 0 1\n";
+        h << "Offset " << x0 << " " << y0 << " " << z0 << "\n";
+        h << "CenterOfRotation 0 0 0\n";
+    } else {
+        h << "TransformMatrix = 1 0 0 0 1 0 0 0 1\n";
+        h << "Origin = " << std::setprecision(9) << x0 << " " << y0 << " " << z0 << "\n";
+        h << "CenterOfRotation = 0 0 0\n";
+    }
+    h << "AnatomicOrientation = RAI\n";
+    h << "DimSize = " << nx << " " << ny << " " << nz << "\n";
+    h << "ElementType = MET_DOUBLE\n";
+    if (mha) h << "HeaderSize = -1\n";
+    h << "ElementSpacing = " << dx << " " << dy << " " << dz << "\n";
+    const std::string path = dir + "/" + name + (mha ? ".mha" : ".mhd");
+    std::ofstream     f(path, std::ios::binary | std::ios::trunc);
+    if (mha) {
+        h << "ElementDataFile = LOCAL\n";
+        const std::string s = h.str();
+        f.write(s.data(), (std::streamsize) s.size());
+        f.write(reinterpret_cast<const char*>(d.data()), (std::streamsize) (n * sizeof(double)));
+    } else {
+        h << "ElementDataFile = " << name << ".raw\n";
+        const std::string s = h.str();
+        f.write(s.data(), (std::streamsize) s.size());
+    }
+    if (!f) throw io_error("error writing " + path);
+    if (!mha) {
+        std::ofstream r(dir + "/" + name + ".raw", std::ios::binary | std::ios::trunc);
+        r.write(reinterpret_cast<const char*>(d.data()), (std::streamsize) (n * sizeof(double)));
+        if (!r) throw io_error("error writing " + dir + "/" + name + ".raw");
+    }
+}
+
+}   // namespace io
+}   // namespace mqi_hip
+
+#endif

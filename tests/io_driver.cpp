@@ -1,0 +1,61 @@
+// Test driver for the output writers (moquimc_amd/host/mqi_hip_io.hpp):
+//   io_driver crc <file>                       prints the CRC-32 of the file's bytes (hex)
+//   io_driver npz <out.npz> <rows> <cols> <scale> <indptr.i64> <indices.i32> <values.f32>
+//   io_driver dense <dir> <name> <nx> <ny> <nz> <scale> <values.f64> <xe.f32> <ye.f32> <ze.f32>
+//       writes <name>_raw.raw, <name>_mhd.mhd + .raw and <name>_mha.mha
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <fstream>
+#include <iterator>
+#include <string>
+#include <vector>
+
+#include "mqi_hip_io.hpp"
+
+template<typename T>
+static std::vector<T>
+slurp(const char* path) {
+    std::ifstream     f(path, std::ios::binary);
+    std::vector<char> b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<T>    v(b.size() / sizeof(T));
+    if (!v.empty()) std::memcpy(v.data(), b.data(), v.size() * sizeof(T));
+    return v;
+}
+
+int
+main(int argc, char** argv) {
+    try {
+        const std::string mode = argc > 1 ? argv[1] : "";
+        if (mode == "crc" && argc == 3) {
+            std::vector<char> b = slurp<char>(argv[2]);
+            std::printf("%08x\n", mqi_hip::io::crc32_update(0, b.data(), b.size()));
+            return 0;
+        }
+        if (mode == "npz" && argc == 9) {
+            auto ip = slurp<int64_t>(argv[6]);
+            auto ix = slurp<int32_t>(argv[7]);
+            auto v  = slurp<float>(argv[8]);
+            mqi_hip::io::save_npz_csr(argv[2], (uint32_t) std::atol(argv[3]), (uint32_t) std::atol(argv[4]), ip.data(),
+                                      ix.data(), v.data(), std::atof(argv[5]));
+            return 0;
+        }
+        if (mode == "dense" && argc == 12) {
+            const int nx = std::atoi(argv[4]), ny = std::atoi(argv[5]), nz = std::atoi(argv[6]);
+            const double scale = std::atof(argv[7]);
+            auto d  = slurp<double>(argv[8]);
+            auto xe = slurp<float>(argv[9]);
+            auto ye = slurp<float>(argv[10]);
+            auto ze = slurp<float>(argv[11]);
+            const std::string dir = argv[2], name = argv[3];
+            mqi_hip::io::save_to_bin(d.data(), d.size(), scale, dir, name + "_raw");
+            mqi_hip::io::save_metaimage(d.data(), nx, ny, nz, xe.data(), ye.data(), ze.data(), scale, dir, name + "_mhd", false);
+            mqi_hip::io::save_metaimage(d.data(), nx, ny, nz, xe.data(), ye.data(), ze.data(), scale, dir, name + "_mha", true);
+            return 0;
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    return 2;
+}

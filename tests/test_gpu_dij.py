@@ -185,3 +185,28 @@ def test_sqrt_sized_tables_match_fixed_tables(ctx):
     assert np.allclose(a.data, b.data, rtol=1e-6, atol=0)
     with pytest.raises(RuntimeError, match="overflowed"):
         ctx.run(ctx.config(seed=12, batched=True, dij=True, dij_slots_sqrt=1, dij_capacity=64))
+
+
+def test_save_npz_and_dense_formats(ctx, tmp_path):
+    """mqi_dij_save_npz / mqi_save_scorer: the reference's .npz (rows =
+    spots, float64 value * scale, uint32 indices) read by scipy equals the
+    device CSR; raw / mhd / mha hold the scorer * scale."""
+    g, sp = _setup(ctx)
+    ctx.run(ctx.config(seed=6, batched=True, dij=True, dij_capacity=1 << 15))
+    ctx.dij_build(1e-4)
+    D = _csr(ctx)
+    path = tmp_path / "beam_0_Dij.npz"
+    ctx.dij_save_npz(str(path), scale=1.5)
+    got = sps.load_npz(path)
+    assert got.shape == D.shape and np.array_equal(got.indptr, D.indptr) and np.array_equal(got.indices, D.indices)
+    assert np.array_equal(got.data, D.data * 1.5)
+    dose = ctx.scorer(0)
+    for fmt in ("raw", "mhd", "mha"):
+        ctx.save_scorer(0, str(tmp_path), "Dose_" + fmt, fmt=fmt, scale=2.0)
+    assert np.array_equal(np.fromfile(tmp_path / "Dose_raw.raw", np.float64), dose * 2.0)
+    assert np.array_equal(np.fromfile(tmp_path / "Dose_mhd.raw", np.float64), dose * 2.0)
+    blob = open(tmp_path / "Dose_mha.mha", "rb").read()
+    start = blob.index(b"ElementDataFile = LOCAL\n") + len(b"ElementDataFile = LOCAL\n")
+    assert np.array_equal(np.frombuffer(blob[start:], np.float64), dose * 2.0)
+    with pytest.raises(Exception):
+        ctx.save_scorer(0, str(tmp_path / "no_such_dir"), "x", fmt="raw")

@@ -1,0 +1,137 @@
+"""Output writers (moquimc_amd/host/mqi_hip_io.hpp) in the reference's
+formats (moqui/base/mqi_io.hpp:165-183,249-320,493-591 and the zip writer
+mqi_sparse_io.hpp:166-400), read back by the readers users have: scipy's
+load_npz, zipfile (which checks every member's CRC-32), numpy.load, and a
+MetaImage header parser.  The CRC-32 is pinned by its published check value
+(crc32("123456789") = 0xCBF43926) and by zlib's on random bytes."""
+import binascii
+import os
+import subprocess
+import zipfile
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("io") / "io_driver")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "moquimc_amd", "host"),
+                    os.path.join(ROOT, "tests", "io_driver.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def _run(*args):
+    subprocess.run([str(a) for a in args], check=True)
+
+
+def test_crc32_check_value_and_zlib(driver, tmp_path):
+    f = tmp_path / "check.bin"
+    f.write_bytes(b"123456789")
+    out = subprocess.run([driver, "crc", str(f)], capture_output=True, text=True, check=True).stdout.strip()
+    assert out == "cbf43926"
+    data = np.random.default_rng(0).integers(0, 256, 100003, dtype=np.uint8).tobytes()
+    f.write_bytes(data)
+    out = subprocess.run([driver, "crc", str(f)], capture_output=True, text=True, check=True).stdout.strip()
+    assert int(out, 16) == binascii.crc32(data)
+
+
+def _csr(rng, rows, cols, density=0.05):
+    m = sps.random(rows, cols, density=density, format="csr", random_state=np.random.RandomState(1),
+                   dtype=np.float32)
+    m.data = (m.data + 0.1).astype(np.float32)
+    m.sort_indices()
+    return m
+
+
+def test_npz_reads_back_with_scipy(driver, tmp_path):
+    m = _csr(np.random.default_rng(1), 37, 513)
+    m.indptr = m.indptr.astype(np.int64)
+    ip, ix, v = tmp_path / "ip.bin", tmp_path / "ix.bin", tmp_path / "v.bin"
+    m.indptr.astype(np.int64).tofile(ip)
+    m.indices.astype(np.int32).tofile(ix)
+    m.data.astype(np.float32).tofile(v)
+    out = tmp_path / "Dij.npz"
+    scale = 2.5e7
+    _run(driver, "npz", out, 37, 513, repr(scale), ip, ix, v)
+    with zipfile.ZipFile(out) as z:
+        assert z.testzip() is None                   # every member's CRC-32 checks
+        assert z.namelist() == ["indices.npy", "indptr.npy", "shape.npy", "data.npy", "format.npy"]
+        assert all(i.compress_type == zipfile.ZIP_STORED for i in z.infolist())
+    with np.load(out) as f:
+        assert f["indices"].dtype == np.uint32 and f["indptr"].dtype == np.uint32
+        assert f["data"].dtype == np.float64 and f["shape"].dtype == np.uint32
+        assert list(f["shape"]) == [37, 513]
+        assert f["format"].item() == b"csr"
+    got = sps.load_npz(out)
+    assert got.shape == (37, 513) and got.format == "csr"
+    assert np.array_equal(got.indptr, m.indptr) and np.array_equal(got.indices, m.indices)
+    assert np.array_equal(got.data, m.data.astype(np.float64) * scale)
+
+
+def test_npz_empty_rows_and_empty_matrix(driver, tmp_path):
+    for rows, cols, dens in ((5, 40, 0.0), (6, 50, 0.2)):
+        m = _csr(None, rows, cols, dens)
+        if dens:
+            m = sps.csr_matrix(m.toarray() * (np.arange(rows)[:, None] % 2))   # odd rows only
+        ip, ix, v = tmp_path / "ip.bin", tmp_path / "ix.bin", tmp_path / "v.bin"
+        m.indptr.astype(np.int64).tofile(ip)
+        m.indices.astype(np.int32).tofile(ix)
+        m.data.astype(np.float32).tofile(v)
+        out = tmp_path / ("m%d.npz" % rows)
+        _run(driver, "npz", out, rows, cols, "1.0", ip, ix, v)
+        got = sps.load_npz(out)
+        assert got.shape == (rows, cols) and got.nnz == m.nnz
+        assert np.array_equal(got.toarray(), m.toarray().astype(np.float64))
+
+
+def _meta(path):
+    """key -> value of a MetaImage header (both 'Key = v' and 'Key v'), and
+    the byte offset where the header ends."""
+    raw = open(path, "rb").read()
+    keys, pos = {}, 0
+    while True:
+        end = raw.index(b"\n", pos)
+        line = raw[pos:end].decode()
+        pos = end + 1
+        k, _, v = line.partition(" = ") if " = " in line else line.partition(" ")
+        keys[k] = v
+        if k == "ElementDataFile":
+            return keys, pos
+
+
+def test_dense_raw_mhd_mha(driver, tmp_path):
+    nx, ny, nz = 7, 5, 3
+    xe = np.linspace(-3.5, 3.5, nx + 1, dtype=np.float32) * np.float32(0.98)
+    ye = np.linspace(-10.0, 0.0, ny + 1, dtype=np.float32)
+    ze = np.linspace(0.0, 6.0, nz + 1, dtype=np.float32)
+    d = np.random.default_rng(2).random(nx * ny * nz)
+    files = []
+    for name, a in (("v", d), ("xe", xe), ("ye", ye), ("ze", ze)):
+        p = tmp_path / (name + ".bin")
+        a.tofile(p)
+        files.append(p)
+    scale = 3.0
+    _run(driver, "dense", tmp_path, "Dose", nx, ny, nz, repr(scale), *files)
+    want = d * scale
+    assert np.array_equal(np.fromfile(tmp_path / "Dose_raw.raw", np.float64), want)
+    keys, _ = _meta(tmp_path / "Dose_mhd.mhd")
+    assert keys["ElementDataFile"] == "Dose_mhd.raw" and keys["ElementType"] == "MET_DOUBLE"
+    assert keys["DimSize"].split() == ["7", "5", "3"]
+    assert keys["TransformMatrix"] == "1 0 0 0 1 0 0 0 1" and keys["BinaryDataByteOrderMSB"] == "False"
+    # spacing = e[1] - e[0], offset = first voxel centre (6 significant digits in .mhd)
+    sp = [float(x) for x in keys["ElementSpacing"].split()]
+    off = [float(x) for x in keys["Offset"].split()]
+    for e, s, o in zip((xe, ye, ze), sp, off):
+        assert abs(s - (e[1] - e[0])) <= 1e-5 * abs(e[1] - e[0])
+        assert abs(o - (e[0] + 0.5 * (e[1] - e[0]))) <= 1e-5 * max(1.0, abs(o))
+    assert np.array_equal(np.fromfile(tmp_path / "Dose_mhd.raw", np.float64), want)
+    keys, pos = _meta(tmp_path / "Dose_mha.mha")
+    assert keys["ElementDataFile"] == "LOCAL" and keys["HeaderSize"] == "-1"
+    org = [float(x) for x in keys["Origin"].split()]
+    assert np.allclose(org, off, rtol=1e-5, atol=1e-5)
+    data = np.frombuffer(open(tmp_path / "Dose_mha.mha", "rb").read()[pos:], np.float64)
+    assert np.array_equal(data, want)
