@@ -194,6 +194,14 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
 
+/* Loads a CT from a MetaImage file with the voxels after the header
+ * (ElementDataFile = LOCAL, float32 HU, tps_env::read_ct_image,
+ * mqi_tps_env.hpp:615-701) and sets it as the grid: edges from Offset (the
+ * first voxel's centre) and ElementSpacing (mqi_tps_env.hpp:469-524), HU
+ * truncated to int16, then as mqi_set_grid_hu with the density scale.  shift:
+ * NULL or the volume shift (mm) added to the first edge of each axis. */
+int mqi_set_grid_ct_mha(mqi_ctx* ctx, const char* path, float density_scale, const float* shift);
+
 /* Writes the Dij of a scenario as the reference's scipy-style CSR .npz
  * (io::save_to_npz, mqi_io.hpp:249-320, zip writer mqi_sparse_io.hpp:166-400):
  * an uncompressed zip of indices.npy and indptr.npy (uint32), shape.npy

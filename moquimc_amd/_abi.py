@@ -87,6 +87,7 @@ def lib():
             "mqi_autotune": (i, [vp, ctypes.POINTER(RunConfig), d, ctypes.POINTER(d)]),
             "mqi_math_probe": (i, [vp, i, vp, vp, ctypes.c_int64, vp]),
             "mqi_dij_save_npz": (i, [vp, i, ctypes.c_char_p, d]),
+            "mqi_set_grid_ct_mha": (i, [vp, ctypes.c_char_p, f, vp]),
             "mqi_save_scorer": (i, [vp, i, i, ctypes.c_char_p, ctypes.c_char_p, i, d]),
         }
         for name, (res, args) in sig.items():
@@ -102,7 +103,7 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
-            "mqi_dij_save_npz", "mqi_save_scorer"]
+            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha"]
 
 FORMATS = {"raw": 0, "mhd": 1, "mha": 2}
 
@@ -172,6 +173,22 @@ class Context:
         trans = None if trans is None else np.ascontiguousarray(trans, np.float32)
         self._chk(self._L.mqi_set_grid_hu(self.h, nx, ny, nz, _p(xe), _p(ye), _p(ze), _p(hu),
                                           ctypes.c_float(density_scale), _p(rot), _p(trans)), "mqi_set_grid_hu")
+        self.nvox = nx * ny * nz
+        self.shape = (nz, ny, nx)
+
+    def set_grid_ct_mha(self, path, density_scale=1.0, shift=None):
+        """CT from a MetaImage file (float32 HU after the header)."""
+        sh = None if shift is None else np.ascontiguousarray(shift, np.float32).reshape(3)
+        self._chk(self._L.mqi_set_grid_ct_mha(self.h, os.fsencode(path), ctypes.c_float(density_scale), _p(sh)),
+                  "mqi_set_grid_ct_mha")
+        nx = ny = nz = 0
+        with open(path, "rb") as f:   # the grid's size, for the download helpers
+            for raw in f:
+                key, _, val = raw.decode("latin-1").partition("=")
+                if key.strip().lower() == "dimsize":
+                    nx, ny, nz = (int(v) for v in val.split()[:3])
+                if key.strip().lower() == "elementdatafile":
+                    break
         self.nvox = nx * ny * nz
         self.shape = (nz, ny, nx)
 

@@ -1318,6 +1318,19 @@ mqi_dij_copy(mqi_ctx* c, int s, int64_t* indptr, int32_t* indices, float* values
 }
 
 int
+mqi_set_grid_ct_mha(mqi_ctx* c, const char* path, float density_scale, const float* shift) {
+    if (!c || !path) return MQI_E_ARG;
+    mqi_hip::io::ct_image ct;
+    try {
+        ct = mqi_hip::io::read_mha_ct(path, shift);
+    } catch (const std::exception& e) {
+        return fail(c, MQI_E_IO, "%s", e.what());
+    }
+    return mqi_set_grid_hu(c, ct.nx, ct.ny, ct.nz, ct.xe.data(), ct.ye.data(), ct.ze.data(), ct.hu.data(),
+                           density_scale, nullptr, nullptr);
+}
+
+int
 mqi_dij_save_npz(mqi_ctx* c, int s, const char* path, double scale) {
     if (!c || !path) return MQI_E_ARG;
     if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);

@@ -19,6 +19,7 @@
 #define MQI_HIP_IO_HPP
 
 #include <array>
+#include <cctype>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -277,6 +278,101 @@ save_metaimage(const double* src, int nx, int ny, int nz, const float* xe, const
         r.write(reinterpret_cast<const char*>(d.data()), (std::streamsize) (n * sizeof(double)));
         if (!r) throw io_error("error writing " + dir + "/" + name + ".raw");
     }
+}
+
+// A CT in MetaImage form with the data in the file (ElementDataFile =
+// LOCAL), as tps_env::read_ct_image reads it (mqi_tps_env.hpp:615-701 and
+// the edge construction :469-524): Offset = centre of the first voxel,
+// ElementSpacing = (dx, dy, dz), DimSize = (nx, ny, nz), float32 voxels
+// converted to int16 HU by truncation; edges x_i = (x0 - dx / 2) + i dx, y
+// likewise, z_i = z_{i-1} + dz from z_0 = z0 - dz / 2, with an optional shift
+// added to the first edge of each axis (the volume shift knobs).  Header keys
+// are matched without regard to case.  Different on purpose: an ElementType
+// other than MET_FLOAT is refused (the reference reads float32 whatever the
+// header says), as are a missing key and a short file.
+struct ct_image {
+    int                  nx = 0, ny = 0, nz = 0;
+    std::array<float, 3> origin{}, spacing{};
+    std::vector<float>   xe, ye, ze;
+    std::vector<int16_t> hu;
+};
+
+inline std::string
+trim(const std::string& s) {
+    const size_t a = s.find_first_not_of(" \t\r\n");
+    if (a == std::string::npos) return "";
+    const size_t b = s.find_last_not_of(" \t\r\n");
+    return s.substr(a, b - a + 1);
+}
+
+inline bool
+iequal(const std::string& a, const char* b) {
+    size_t n = 0;
+    for (; b[n]; ++n)
+        if (n >= a.size() || std::tolower((unsigned char) a[n]) != std::tolower((unsigned char) b[n])) return false;
+    return n == a.size();
+}
+
+inline ct_image
+read_mha_ct(const std::string& path, const float* shift = nullptr) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw io_error("cannot open " + path);
+    ct_image    ct;
+    bool        have_off = false, have_sp = false, have_dim = false, local = false;
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string key = trim(line.substr(0, eq));
+        std::istringstream v(trim(line.substr(eq + 1)));
+        // numbers are read as double and rounded to float, as atof into a
+        // float does
+        double a, b, c;
+        if (iequal(key, "Offset")) {
+            have_off  = (bool) (v >> a >> b >> c);
+            ct.origin = {(float) a, (float) b, (float) c};
+        } else if (iequal(key, "ElementSpacing")) {
+            have_sp    = (bool) (v >> a >> b >> c);
+            ct.spacing = {(float) a, (float) b, (float) c};
+        } else if (iequal(key, "DimSize")) {
+            have_dim = (bool) (v >> ct.nx >> ct.ny >> ct.nz);
+        } else if (iequal(key, "ElementType")) {
+            std::string t;
+            v >> t;
+            if (t != "MET_FLOAT") throw io_error(path + ": ElementType " + t + " (the CT reader takes MET_FLOAT)");
+        } else if (iequal(key, "ElementDataFile")) {
+            std::string t;
+            v >> t;
+            if (!iequal(t, "LOCAL")) throw io_error(path + ": the data must follow the header (ElementDataFile = LOCAL)");
+            local = true;
+            break;
+        }
+    }
+    if (!have_off || !have_sp || !have_dim || !local) throw io_error(path + ": Offset, ElementSpacing, DimSize or data missing");
+    if (ct.nx <= 0 || ct.ny <= 0 || ct.nz <= 0) throw io_error(path + ": bad DimSize");
+    const size_t       n = (size_t) ct.nx * ct.ny * ct.nz;
+    std::vector<float> raw(n);
+    f.read(reinterpret_cast<char*>(raw.data()), (std::streamsize) (n * sizeof(float)));
+    if ((size_t) f.gcount() != n * sizeof(float)) throw io_error(path + ": fewer voxels than DimSize");
+    ct.hu.resize(n);
+    for (size_t i = 0; i < n; ++i) ct.hu[i] = (int16_t) raw[i];
+    const float dx = ct.spacing[0], dy = ct.spacing[1], dz = ct.spacing[2];
+    float       x0 = ct.origin[0] - dx / 2.0;
+    float       y0 = ct.origin[1] - dy / 2.0;
+    float       z0 = ct.origin[2] - dz / 2.0;
+    if (shift) {
+        x0 += shift[0];
+        y0 += shift[1];
+        z0 += shift[2];
+    }
+    ct.xe.resize(ct.nx + 1);
+    ct.ye.resize(ct.ny + 1);
+    ct.ze.resize(ct.nz + 1);
+    for (int i = 0; i <= ct.nx; ++i) ct.xe[i] = x0 + i * dx;
+    for (int i = 0; i <= ct.ny; ++i) ct.ye[i] = y0 + i * dy;
+    ct.ze[0] = z0;
+    for (int i = 1; i <= ct.nz; ++i) ct.ze[i] = ct.ze[i - 1] + dz;
+    return ct;
 }
 
 }   // namespace io

@@ -1,6 +1,7 @@
 // Test driver for the output writers (moquimc_amd/host/mqi_hip_io.hpp):
 //   io_driver crc <file>                       prints the CRC-32 of the file's bytes (hex)
 //   io_driver npz <out.npz> <rows> <cols> <scale> <indptr.i64> <indices.i32> <values.f32>
+//   io_driver ct <ct.mha> <hu.out> [sx sy sz]  reads a MetaImage CT, prints dims and edges
 //   io_driver dense <dir> <name> <nx> <ny> <nz> <scale> <values.f64> <xe.f32> <ye.f32> <ze.f32>
 //       writes <name>_raw.raw, <name>_mhd.mhd + .raw and <name>_mha.mha
 #include <cstdio>
@@ -38,6 +39,21 @@ main(int argc, char** argv) {
             auto v  = slurp<float>(argv[8]);
             mqi_hip::io::save_npz_csr(argv[2], (uint32_t) std::atol(argv[3]), (uint32_t) std::atol(argv[4]), ip.data(),
                                       ix.data(), v.data(), std::atof(argv[5]));
+            return 0;
+        }
+        if (mode == "ct" && (argc == 4 || argc == 7)) {
+            // prints nx ny nz, then the edges as hex floats; HU to <out>
+            float sh[3];
+            if (argc == 7)
+                for (int k = 0; k < 3; ++k) sh[k] = (float) std::atof(argv[4 + k]);
+            mqi_hip::io::ct_image ct = mqi_hip::io::read_mha_ct(argv[2], argc == 7 ? sh : nullptr);
+            std::printf("%d %d %d\n", ct.nx, ct.ny, ct.nz);
+            for (const std::vector<float>* e : {&ct.xe, &ct.ye, &ct.ze}) {
+                for (float v : *e) std::printf("%a ", v);
+                std::printf("\n");
+            }
+            std::ofstream o(argv[3], std::ios::binary);
+            o.write(reinterpret_cast<const char*>(ct.hu.data()), (std::streamsize) (ct.hu.size() * 2));
             return 0;
         }
         if (mode == "dense" && argc == 12) {

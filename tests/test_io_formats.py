@@ -135,3 +135,62 @@ def test_dense_raw_mhd_mha(driver, tmp_path):
     assert np.allclose(org, off, rtol=1e-5, atol=1e-5)
     data = np.frombuffer(open(tmp_path / "Dose_mha.mha", "rb").read()[pos:], np.float64)
     assert np.array_equal(data, want)
+
+
+def _write_ct(path, hu, origin, spacing, element_type="MET_FLOAT", key_case=str):
+    nz, ny, nx = hu.shape
+    hdr = ["ObjectType = Image", "NDims = 3", "BinaryData = True", "BinaryDataByteOrderMSB = False",
+           "CompressedData = False", "TransformMatrix = 1 0 0 0 1 0 0 0 1",
+           key_case("Offset") + " = %r %r %r" % tuple(origin), "CenterOfRotation = 0 0 0",
+           key_case("ElementSpacing") + " = %r %r %r" % tuple(spacing), "DimSize = %d %d %d" % (nx, ny, nz),
+           "ElementType = " + element_type, "ElementDataFile = LOCAL"]
+    with open(path, "wb") as f:
+        f.write(("\n".join(hdr) + "\n").encode())
+        f.write(hu.astype(np.float32).tobytes())
+
+
+def _edges(o, d, n, shift=0.0):
+    # x0 = origin - d / 2.0 in double, rounded to float; + shift in float;
+    # x_i = x0 + i * d in float
+    f32 = np.float32
+    x0 = f32(f32(o) - f32(d) / 2.0)
+    x0 = f32(x0 + f32(shift))
+    return np.array([f32(x0 + f32(f32(i) * f32(d))) for i in range(n + 1)], np.float32)
+
+
+def test_mha_ct_reader(driver, tmp_path):
+    rng = np.random.default_rng(3)
+    hu = rng.uniform(-1000, 2000, (4, 5, 6)).astype(np.float32)   # non-integer values: truncation
+    origin, spacing = (-10.25, 3.5, -100.0), (0.98, 1.25, 2.0)
+    p = tmp_path / "ct.mha"
+    _write_ct(p, hu, origin, spacing, key_case=str.upper)
+    out = subprocess.run([driver, "ct", str(p), str(tmp_path / "hu.bin")], capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    assert out[0].split() == ["6", "5", "4"]
+    xe, ye, ze = ([float.fromhex(v) for v in ln.split()] for ln in out[1:4])
+    assert np.array_equal(np.float32(xe), _edges(origin[0], spacing[0], 6))
+    assert np.array_equal(np.float32(ye), _edges(origin[1], spacing[1], 5))
+    # z accumulates: z_i = z_{i-1} + dz
+    z = [np.float32(np.float32(origin[2]) - np.float32(spacing[2]) / 2.0)]
+    for _ in range(4):
+        z.append(np.float32(z[-1] + np.float32(spacing[2])))
+    assert np.array_equal(np.float32(ze), np.float32(z))
+    # the first voxel centre is the Offset, to rounding
+    assert abs(0.5 * (xe[0] + xe[1]) - origin[0]) < 1e-5
+    got = np.fromfile(tmp_path / "hu.bin", np.int16)
+    assert np.array_equal(got, np.trunc(hu).astype(np.int16).ravel())
+    # a volume shift moves the first edge
+    out = subprocess.run([driver, "ct", str(p), str(tmp_path / "hu.bin"), "1.5", "-2", "0"], capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    assert np.array_equal(np.float32([float.fromhex(v) for v in out[1].split()]), _edges(origin[0], spacing[0], 6, 1.5))
+
+
+def test_mha_ct_reader_refuses_bad_files(driver, tmp_path):
+    hu = np.zeros((2, 2, 2), np.float32)
+    p = tmp_path / "bad.mha"
+    _write_ct(p, hu, (0, 0, 0), (1, 1, 1), element_type="MET_SHORT")
+    assert subprocess.run([driver, "ct", str(p), str(tmp_path / "o.bin")], capture_output=True).returncode == 1
+    _write_ct(p, hu, (0, 0, 0), (1, 1, 1))
+    data = open(p, "rb").read()
+    open(p, "wb").write(data[:-4])   # one voxel short
+    assert subprocess.run([driver, "ct", str(p), str(tmp_path / "o.bin")], capture_output=True).returncode == 1
