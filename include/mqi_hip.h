@@ -194,6 +194,19 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
 
+/* Beamline children transported before the patient grid, in order (the
+ * world of tps_env::setup_world, mqi_tps_env.hpp:713-753: range shifter and
+ * voxelised aperture nodes, then the phantom; transport loop
+ * mqi_transport.hpp:158-240).  Every track (primary or popped secondary)
+ * goes through the children from the first: into the child's frame, to its
+ * first voxel, steps with the same physics until it leaves or stops, back to
+ * the world frame; then the next child, the patient last.  Beamline children
+ * have no scorers; an aperture blocks with density above 99.9 g/mm^3
+ * (stepping stops the track), passes with density below 1e-7.  Robust
+ * scenarios shift and scale the patient only.  n = 0 removes them; at most
+ * 8 children. */
+int mqi_set_beamline(mqi_ctx* ctx, int n, const mqi_grid_desc* children);
+
 /* Loads a CT from a MetaImage file with the voxels after the header
  * (ElementDataFile = LOCAL, float32 HU, tps_env::read_ct_image,
  * mqi_tps_env.hpp:615-701) and sets it as the grid: edges from Offset (the

@@ -9,6 +9,22 @@
 extern "C" {
 #endif
 
+/* A voxel grid placed in the world: a beamline child (range shifter,
+ * aperture) of mqi_set_beamline.  Edges ascend, n + 1 per axis, in the
+ * child's frame; density in g/mm^3, x fastest; rotation (row-major) and
+ * translation map the child's frame to the world (grid3d's
+ * rotation_matrix_fwd and translation_vector, mqi_grid3d.hpp:31-890). */
+typedef struct mqi_grid_desc {
+    int32_t      nx, ny, nz;
+    int32_t      reserved;
+    const float* xe;
+    const float* ye;
+    const float* ze;
+    const float* density;
+    float        rotation[9];
+    float        translation[3];
+} mqi_grid_desc;
+
 /* One pencil-beam spot (a "beamlet" in the reference).  The phase-space and
  * energy model is the reference's phsp_6d + norm_1d/const_1d beamlet
  * (moqui/base/distributions/mqi_phsp6d.hpp:62-79, mqi_norm_1d.hpp:54-56,

@@ -88,6 +88,7 @@ def lib():
             "mqi_math_probe": (i, [vp, i, vp, vp, ctypes.c_int64, vp]),
             "mqi_dij_save_npz": (i, [vp, i, ctypes.c_char_p, d]),
             "mqi_set_grid_ct_mha": (i, [vp, ctypes.c_char_p, f, vp]),
+            "mqi_set_beamline": (i, [vp, i, vp]),
             "mqi_save_scorer": (i, [vp, i, i, ctypes.c_char_p, ctypes.c_char_p, i, d]),
         }
         for name, (res, args) in sig.items():
@@ -103,7 +104,14 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
-            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha"]
+            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline"]
+
+
+class GridDesc(ctypes.Structure):
+    """mqi_grid_desc: a beamline child grid (include/mqi_hip_types.h)."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("xe", ctypes.c_void_p), ("ye", ctypes.c_void_p), ("ze", ctypes.c_void_p),
+                ("density", ctypes.c_void_p), ("rotation", ctypes.c_float * 9), ("translation", ctypes.c_float * 3)]
 
 FORMATS = {"raw": 0, "mhd": 1, "mha": 2}
 
@@ -175,6 +183,25 @@ class Context:
                                           ctypes.c_float(density_scale), _p(rot), _p(trans)), "mqi_set_grid_hu")
         self.nvox = nx * ny * nz
         self.shape = (nz, ny, nx)
+
+    def set_beamline(self, children=()):
+        """Beamline children before the patient, in order; each has xe, ye,
+        ze (edges in its frame), density (g/mm^3, x fastest) and optional
+        rot9 (frame -> world) and t3 (translation).  No arguments: none."""
+        keep, descs = [], (GridDesc * max(1, len(children)))()
+        for k, g in enumerate(children):
+            xe, ye, ze = (np.ascontiguousarray(a, np.float32) for a in (g.xe, g.ye, g.ze))
+            dens = np.ascontiguousarray(g.density, np.float32).reshape(-1)
+            keep += [xe, ye, ze, dens]
+            d = descs[k]
+            d.nx, d.ny, d.nz = len(xe) - 1, len(ye) - 1, len(ze) - 1
+            d.xe, d.ye, d.ze, d.density = (a.ctypes.data for a in (xe, ye, ze, dens))
+            rot = getattr(g, "rot9", None)
+            tr = getattr(g, "t3", None)
+            d.rotation[:] = list(np.asarray(rot if rot is not None else np.eye(3), np.float32).ravel())
+            d.translation[:] = list(np.asarray(tr if tr is not None else np.zeros(3), np.float32).ravel())
+        self._chk(self._L.mqi_set_beamline(self.h, len(children), ctypes.cast(descs, ctypes.c_void_p)),
+                  "mqi_set_beamline")
 
     def set_grid_ct_mha(self, path, density_scale=1.0, shift=None):
         """CT from a MetaImage file (float32 HU after the header)."""

@@ -102,6 +102,14 @@ struct mqi_ctx {
     uint32_t* d_item_scen = nullptr;
     size_t    items_cap   = 0;
 
+    // beamline children (device copies; see transport_params)
+    int       bl_n       = 0;
+    int       bl_n_edges = 0;
+    int*      d_bl_meta  = nullptr;
+    float*    d_bl_xf    = nullptr;
+    float*    d_bl_edges = nullptr;
+    float*    d_bl_dens  = nullptr;
+
     // Dij
     uint32_t*             d_dij_keys  = nullptr;
     double*               d_dij_vals  = nullptr;
@@ -285,7 +293,7 @@ upload_grid_common(mqi_ctx* c, int nx, int ny, int nz, const float* xe, const fl
 
 size_t
 shmem_bytes(const mqi_ctx* c) {
-    return sizeof(float) * (3600 + (size_t) c->nx + 1 + c->ny + 1 + c->nz + 1);
+    return sizeof(float) * (3600 + (size_t) c->nx + 1 + c->ny + 1 + c->nz + 1 + c->bl_n_edges);
 }
 
 int
@@ -364,6 +372,12 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.status        = c->d_status;
     P.hist_offset   = c->d_hoff;
     P.seed          = cfg.seed;
+    P.n_bl          = c->bl_n;
+    P.bl_meta       = c->d_bl_meta;
+    P.bl_xf         = c->d_bl_xf;
+    P.bl_edges      = c->d_bl_edges;
+    P.bl_n_edges    = c->bl_n_edges;
+    P.bl_density    = c->d_bl_dens;
     // alias_dose: the Dose scorer is derived from Dose_stat after the run
     // (the kernel then does one atomic less per step)
     P.n_scorers = 0;
@@ -409,7 +423,7 @@ launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, 
     for (int s = 0; s < P.n_scorers; ++s)
         if (P.kinds[s] == MQI_KIND_DIJ ? (!P.dij_keys || !P.dij_vals) : !P.out[s]) return (int) hipErrorInvalidValue;
     if (trace && (!P.trace_cnb || !P.trace_rec || !P.trace_n)) return (int) hipErrorInvalidValue;
-    if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * 9 * sizeof(float) > c->stack_bytes)
+    if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float) > c->stack_bytes)
         return (int) hipErrorInvalidValue;
     const size_t sh = shmem_bytes(c);
     if (dynamic) {
@@ -516,7 +530,7 @@ transport_pass(mqi_ctx* c, const mqi_run_config& cfg, uint64_t hoff, double* ms)
     std::memset(&P, 0, sizeof P);   // defined padding: the struct is compared byte-wise for graph reuse
     fill_params(c, P, cfg);
     KCHK(c, mqi_aux::fill_u64(c->d_hoff, hoff, 1, c->stream));
-    const size_t lane_bytes = (size_t) P.stack_cap * 9 * sizeof(float);
+    const size_t lane_bytes = (size_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float);
     int          r;
     if (cfg.batched) {
         run_plan p = make_plan(c, -1);
@@ -721,6 +735,10 @@ mqi_destroy(mqi_ctx* c) {
     dfree(c->d_dij_keys);
     dfree(c->d_dij_vals);
     dfree(c->d_dij_off);
+    dfree(c->d_bl_meta);
+    dfree(c->d_bl_xf);
+    dfree(c->d_bl_edges);
+    dfree(c->d_bl_dens);
     free_dij(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -1090,7 +1108,7 @@ mqi_run_vertices(mqi_ctx* c, const float* vtx, uint64_t n, const uint32_t* spot_
         uint64_t lanes = std::min<uint64_t>(n, 1ull << 18);
         blocks         = (int) ((lanes + threads - 1) / threads);
     }
-    if ((r = ensure_stack(c, (size_t) P.stack_cap * 9 * sizeof(float) * (size_t) blocks * threads))) return r;
+    if ((r = ensure_stack(c, (size_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float) * (size_t) blocks * threads))) return r;
     P.stack = c->d_stack;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     KCHK(c, launch(c, P, cfg, cfg.batched ? 1 : 0, 0, blocks, threads));
@@ -1159,7 +1177,7 @@ mqi_trace(mqi_ctx* c, const mqi_run_config* cfg_in, const float* vertices, uint6
     KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 4, c->stream));
     const int threads = 64;
     const int blocks  = (int) ((n_hist + threads - 1) / threads);
-    if ((r = ensure_stack(c, (size_t) P.stack_cap * 9 * sizeof(float) * (size_t) blocks * threads))) return r;
+    if ((r = ensure_stack(c, (size_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float) * (size_t) blocks * threads))) return r;
     P.stack = c->d_stack;
     KCHK(c, launch(c, P, cfg, 0, 1, blocks, threads));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1314,6 +1332,52 @@ mqi_dij_copy(mqi_ctx* c, int s, int64_t* indptr, int32_t* indices, float* values
     if (indptr) HIPCHK(c, hipMemcpy(indptr, m.indptr, sizeof(int64_t) * (m.rows + 1), hipMemcpyDeviceToHost));
     if (indices && m.nnz) HIPCHK(c, hipMemcpy(indices, m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
     if (values && m.nnz) HIPCHK(c, hipMemcpy(values, m.val, sizeof(float) * m.nnz, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int
+mqi_set_beamline(mqi_ctx* c, int n, const mqi_grid_desc* d) {
+    if (!c) return MQI_E_ARG;
+    if (n < 0 || n > 8) return fail(c, MQI_E_ARG, "beamline children: %d (0 to 8)", n);
+    if (n > 0 && !d) return fail(c, MQI_E_ARG, "children is NULL");
+    hipSetDevice(c->device);
+    std::vector<int>   meta;
+    std::vector<float> xf, edges, dens;
+    for (int k = 0; k < n; ++k) {
+        const mqi_grid_desc& g = d[k];
+        if (!g.xe || !g.ye || !g.ze || !g.density) return fail(c, MQI_E_ARG, "child %d: NULL array", k);
+        int r;
+        if ((r = check_edges(c, g.xe, g.nx, "beamline x")) || (r = check_edges(c, g.ye, g.ny, "beamline y")) ||
+            (r = check_edges(c, g.ze, g.nz, "beamline z")))
+            return r;
+        const uint64_t nv = (uint64_t) g.nx * g.ny * g.nz;
+        if (nv >= (1ull << 31)) return fail(c, MQI_E_ARG, "child %d: more than 2^31 voxels", k);
+        const uint64_t doff = dens.size();
+        meta.insert(meta.end(), {g.nx, g.ny, g.nz, (int) edges.size(), (int) (uint32_t) doff, (int) (uint32_t) (doff >> 32), 0, 0});
+        edges.insert(edges.end(), g.xe, g.xe + g.nx + 1);
+        edges.insert(edges.end(), g.ye, g.ye + g.ny + 1);
+        edges.insert(edges.end(), g.ze, g.ze + g.nz + 1);
+        dens.insert(dens.end(), g.density, g.density + nv);
+        xf.insert(xf.end(), g.rotation, g.rotation + 9);
+        // grid3d::rotation_matrix_inv = fwd.inverse() (the transpose, mqi_matrix.hpp:290-293)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) xf.push_back(g.rotation[3 * j + i]);
+        xf.insert(xf.end(), g.translation, g.translation + 3);
+    }
+    if (edges.size() > 16384) return fail(c, MQI_E_ARG, "beamline edges: %zu (at most 16384 staged in LDS)", edges.size());
+    int r;
+    c->bl_n       = 0;
+    c->bl_n_edges = 0;
+    if (n == 0) return 0;
+    if ((r = dalloc(c, &c->d_bl_meta, meta.size())) || (r = dalloc(c, &c->d_bl_xf, xf.size())) ||
+        (r = dalloc(c, &c->d_bl_edges, edges.size())) || (r = dalloc(c, &c->d_bl_dens, dens.size())))
+        return r;
+    HIPCHK(c, hipMemcpy(c->d_bl_meta, meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_bl_xf, xf.data(), sizeof(float) * xf.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_bl_edges, edges.data(), sizeof(float) * edges.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_bl_dens, dens.data(), sizeof(float) * dens.size(), hipMemcpyHostToDevice));
+    c->bl_n       = n;
+    c->bl_n_edges = (int) edges.size();
     return 0;
 }
 

@@ -480,7 +480,10 @@ MQI_DI void rotate_direction(v3& d, float theta, float phi) {
         float dot_uv = vv.x * uu.x + vv.y * uu.y + vv.z * uu.z;
         float U[3]   = { uu.x, uu.y, uu.z };
         float V[3]   = { vv.x, vv.y, vv.z };
+        // fully unrolled: a runtime index into g, U or V would put them in private memory
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j)
                 g[3 * i + j] = (i == j ? 1 : 0) - 2 / dot_u * U[i] * U[j] - 2 / dot_v * V[i] * V[j] +
                                4 * dot_uv / (dot_u * dot_v) * V[i] * U[j];

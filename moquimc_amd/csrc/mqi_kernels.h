@@ -26,6 +26,10 @@ enum {
     MQI_LOOPS_STAT  = MQI_LOOPS_REFERENCE_STAT     // (mqi_transport.hpp:343-364)
 };
 
+// floats per secondary-stack entry: ke0, pos[3], dir[3], ke1, dE, and the
+// beamline child whose transform still has to place it (-1: placed)
+#define MQI_STACK_FLOATS 10
+
 #define MQI_STATUS_DIJ_FULL 1u
 #define MQI_STATUS_TRACK_CAP 2u
 #define MQI_MAX_TRACK_STEPS 200000
@@ -87,6 +91,17 @@ struct transport_params {
     unsigned long long* drops;
     unsigned long long* steps;   // voxel steps taken (all tracks)
     unsigned int*       status;
+    // beamline children transported before the patient, in order (range
+    // shifter, aperture, ...): per child bl_meta = {nx, ny, nz, offset of its
+    // edges in bl_edges (x, y, z concatenated), offset of its densities in
+    // bl_density (two ints, low and high 32 bits), 0} and bl_xf = {fwd[9],
+    // inv[9], translation[3]}; no scorers
+    int          n_bl;
+    const int*   bl_meta;
+    const float* bl_xf;
+    const float* bl_edges;
+    int          bl_n_edges;
+    const float* bl_density;
     // per-step trace (verification)
     int64_t* trace_cnb;
     float*   trace_rec;

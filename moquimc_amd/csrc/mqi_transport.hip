@@ -37,12 +37,15 @@ struct ctx {
     const float* T;   // LDS tables
     grid_view    g;
     rng_t        rng;
-    float*       stack;   // HBM, stack_cap * 9 floats for this lane
+    float*       stack;   // HBM, stack_cap * MQI_STACK_FLOATS floats for this lane
     int          sp;
     int          cap;
     float        fwd[9];   // held by value: a pointer into the kernel
     float        trans[3]; // arguments would force them into private memory
     unsigned     drops;
+    int          child;    // world child the track is in: 0 .. nbl - 1 beamline, nbl the patient
+    int          nbl;      // beamline children
+    const float* blxf;     // their transforms (fwd[9], inv[9], translation[3] each), HBM
 };
 
 // p_ionization_tabulated::cross_section (mqi_p_ionization.hpp:254-268): Ei=0.1, Ef=299.6, step 0.5
@@ -141,7 +144,12 @@ MQI_DI void along_step(ctx& c, trk& t, float len, float rho, float sp) {
     t.ke1 -= dE * r;
 }
 
-// secondary placement: pos -> fwd (pos - T) + T, dir -> fwd dir (e.g. mqi_pp_elastic.hpp:190-196)
+// secondary placement: pos -> fwd (pos - T) + T, dir -> fwd dir (e.g.
+// mqi_pp_elastic.hpp:190-196), with the transform of the child the track is
+// in.  In the patient (c.fwd, c.trans) at the push; a secondary made in a
+// beamline child is stored unplaced with the child's index and placed with
+// that child's transform when it is popped (the same operations, so the same
+// bits), which keeps the beamline transforms out of the physics.
 MQI_DI v3 place(const ctx& c, v3 p) {
     MQI_NO_CONTRACT
     v3 q = mk(p.x - c.trans[0], p.y - c.trans[1], p.z - c.trans[2]);
@@ -149,13 +157,19 @@ MQI_DI v3 place(const ctx& c, v3 p) {
     return mk(q.x + c.trans[0], q.y + c.trans[1], q.z + c.trans[2]);
 }
 MQI_DI v3 turn(const ctx& c, v3 d) { return mv(c.fwd, d); }
+// the same with a child's transform (fwd[9], inv[9], translation[3]) in HBM
+MQI_DI v3 place_xf(const float* f, v3 p) {
+    MQI_NO_CONTRACT
+    v3 q = mv(f, mk(p.x - f[18], p.y - f[19], p.z - f[20]));
+    return mk(q.x + f[18], q.y + f[19], q.z + f[20]);
+}
 
 // track_stack_t::push_secondary: dropped when the stack is full (mqi_track_stack.hpp:36-41).
 // A pushed track keeps vtx0, vtx1.ke and dE; vtx1 position/direction are
 // overwritten when it is popped, local_dE is always zero, status is CREATED.
-MQI_DI void push(ctx& c, float ke0, v3 p0, v3 d0, float ke1, float dE) {
+MQI_DI void push(ctx& c, float ke0, v3 p0, v3 d0, float ke1, float dE, float tag) {
     if (c.sp < c.cap) {
-        float* s = c.stack + 9 * c.sp;
+        float* s = c.stack + MQI_STACK_FLOATS * c.sp;
         s[0]     = ke0;
         s[1]     = p0.x;
         s[2]     = p0.y;
@@ -165,10 +179,19 @@ MQI_DI void push(ctx& c, float ke0, v3 p0, v3 d0, float ke1, float dE) {
         s[6]     = d0.z;
         s[7]     = ke1;
         s[8]     = dE;
+        s[9]     = tag;
         ++c.sp;
     } else {
         ++c.drops;
     }
+}
+
+// push a daughter made at (p, d) in the current child
+MQI_DI void push_placed(ctx& c, float ke0, v3 p, v3 d, float ke1, float dE) {
+    if (c.child < c.nbl)
+        push(c, ke0, p, d, ke1, dE, (float) c.child);
+    else
+        push(c, ke0, place(c, p), turn(c, d), ke1, dE, -1.0f);
 }
 
 // p_ionization_tabulated::post_step: delta-ray (mqi_p_ionization.hpp:424-477)
@@ -186,7 +209,7 @@ MQI_DI void post_ion(ctx& c, trk& t) {
         if (c.rng.uniform() < 1.0 - rel.beta_sq * Te / rel.Te_max + Te * Te / (2.0 * rel.Et_sq)) break;
     }
     if (DBG) {
-        push(c, 0.0f, place(c, t.p0), turn(c, t.d0), 0.0f, Te);
+        push_placed(c, 0.0f, t.p0, t.d0, 0.0f, Te);
     } else {
         t.dE += Te;
     }
@@ -224,7 +247,7 @@ MQI_DI void post_pp(ctx& c, trk& t) {
     rotate_direction(t.d1, th3, phi);
     v3 dd = t.d1;
     rotate_direction(dd, th4, phi);
-    push(c, dE, place(c, t.p1), turn(c, dd), dE, 0.0f);
+    push_placed(c, dE, t.p1, dd, dE, 0.0f);
 }
 
 // po_elastic::post_step (mqi_po_elastic.hpp:96-217); exponential not truncated (CPU semantics)
@@ -234,7 +257,7 @@ MQI_DI void post_poe(ctx& c, trk& t) {
     if (rel.Ek <= 5.5f) {
         float dE = rel.Ek;
         if (DBG)
-            push(c, dE, place(c, t.p1), turn(c, t.d1), 0.0f, dE);
+            push_placed(c, dE, t.p1, t.d1, 0.0f, dE);
         else
             t.ldE += dE;
         t.ke1 -= dE;
@@ -256,7 +279,7 @@ MQI_DI void post_poe(ctx& c, trk& t) {
         float th3 = MQ_ACOSF(cos_th3);
         float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
         if (DBG)
-            push(c, dE, place(c, t.p0), turn(c, t.d0), 0.0f, dE);
+            push_placed(c, dE, t.p0, t.d0, 0.0f, dE);
         else
             t.ldE += dE;
         t.ke1 -= dE;
@@ -305,11 +328,11 @@ MQI_DI void post_poi(ctx& c, trk& t) {
             float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
             v3    dd  = t.d1;
             rotate_direction(dd, th, phi);
-            push(c, dE, place(c, t.p1), turn(c, dd), dE, 0.0f);
+            push_placed(c, dE, t.p1, dd, dE, 0.0f);
         } else if (zeta < Prob_long) {
             // long-range energy leaves the track
         } else if (DBG) {
-            push(c, dE, place(c, t.p0), turn(c, t.d0), 0.0f, dE);
+            push_placed(c, dE, t.p0, t.d0, 0.0f, dE);
         }
         Eb *= E_ratio;
     }
@@ -554,6 +577,7 @@ struct lane {
     uint64_t dij_item;   // Dij table of (scenario, spot): scenario * n_spots + spot
     int      tsteps;     // steps of the current track
     pending  acc;        // deferred deposits of the current voxel
+    bool     reenter;    // left a beamline child: enter the next child
 };
 
 // pop + enter the child grid: transport loop head (mqi_transport.hpp:158-191)
@@ -584,12 +608,89 @@ MQI_DI bool enter(const transport_params& P, ctx& c, trk& t) {
     return is_valid(c.g, t.cx, t.cy, t.cz) && !t.stopped;
 }
 
+// ------------------------------------------------------------------ beamline children
+// The world's children are transported in order, the patient last
+// (mqi_transport.hpp:158-240): the track goes into the child's frame, to its
+// first voxel, steps until it leaves or stops, and back to the world frame
+// (also when it misses the child) before the next child.  Beamline children
+// have no scorers and no scenario shift or density scale; their edges sit in
+// LDS after the patient's.
+MQI_DI grid_view bl_view(const transport_params& P, const float* shb, int k) {
+    const int* m = P.bl_meta + 8 * k;
+    grid_view  g;
+    g.nx = m[0];
+    g.ny = m[1];
+    g.nz = m[2];
+    g.xe = shb + m[3];
+    g.ye = g.xe + g.nx + 1;
+    g.ze = g.ye + g.ny + 1;
+    g.sx = g.sy = g.sz = 0.0f;
+    return g;
+}
+
+// into child k's frame and to its first voxel (as enter() for the patient)
+MQI_DI bool enter_bl(const transport_params& P, const grid_view& g, int k, trk& t) {
+    MQI_NO_CONTRACT
+    const float* f = P.bl_xf + 21 * k;
+    v3           pp = mk(t.p0.x - f[18], t.p0.y - f[19], t.p0.z - f[20]);
+    t.p0            = mv(f + 9, pp);
+    t.d0            = mv(f + 9, t.d0);
+    normalize(t.d0);
+    t.p1 = t.p0;
+    t.d1 = t.d0;
+    int cx, cy, cz;
+    index3(g, t.p0, t.d0, cx, cy, cz);
+    if (!is_valid(g, cx, cy, cz)) {
+        int   ox, oy, oz;
+        float dist = intersect_out(g, t.p0, t.d0, ox, oy, oz);
+        t.its      = dist;
+        if (dist < 0) return false;
+        update_pos(t, dist);
+        move(t);
+        index3(g, t.p0, t.d0, t.cx, t.cy, t.cz);
+    } else {
+        t.its = 0.0f;
+        t.cx  = cx;
+        t.cy  = cy;
+        t.cz  = cz;
+    }
+    return is_valid(g, t.cx, t.cy, t.cz) && !t.stopped;
+}
+
+// back from child k's frame to the world: pos -> fwd pos + T, dir -> fwd dir
+MQI_DI void leave_bl(const transport_params& P, int k, trk& t) {
+    MQI_NO_CONTRACT
+    const float* f = P.bl_xf + 21 * k;
+    v3           q = mv(f, t.p0);
+    t.p0           = mk(q.x + f[18], q.y + f[19], q.z + f[20]);
+    t.d0           = mv(f, t.d0);
+    t.p1 = t.p0;
+    t.d1 = t.d0;
+}
+
+// the track (in the world frame) enters the first child it reaches from
+// c.child on; false when it reaches none
+MQI_DI bool enter_chain(const transport_params& P, ctx& c, trk& t, const float* shb) {
+    while (c.child < c.nbl) {
+        const grid_view g = bl_view(P, shb, c.child);
+        if (enter_bl(P, g, c.child, t)) return true;
+        leave_bl(P, c.child, t);
+        ++c.child;
+    }
+    return enter(P, c, t);
+}
+
 MQI_DI void pop(ctx& c, trk& t) {
     --c.sp;
-    const float* s = c.stack + 9 * c.sp;
+    const float* s = c.stack + MQI_STACK_FLOATS * c.sp;
     t.ke0          = s[0];
     t.p0           = mk(s[1], s[2], s[3]);
     t.d0           = mk(s[4], s[5], s[6]);
+    if (s[9] >= 0.0f) {   // made in beamline child s[9]: place it now
+        const float* f = c.blxf + 21 * (int) s[9];
+        t.p0           = place_xf(f, t.p0);
+        t.d0           = mv(f, t.d0);
+    }
     t.ke1          = s[7];
     t.dE           = s[8];
     t.ldE          = 0.0f;
@@ -680,15 +781,55 @@ MQI_DI void trace_record(const transport_params& P, lane& L, uint64_t cnb) {
     P.trace_n[g] = L.nsteps;
 }
 
-// one voxel-loop iteration (mqi_transport.hpp:192-233)
+// one voxel-loop iteration (mqi_transport.hpp:192-233), in the patient or
+// in a beamline child (no scoring there; leaving it hands the track to the
+// next child).  One stepping call site serves both.
 template<bool DBG, bool TRACE>
-MQI_DI void step(const transport_params& P, ctx& c, lane& L) {
-    trk&           t   = L.t;
-    const uint64_t cnb = (uint64_t) (int32_t) (t.cz * c.g.nx * c.g.ny + t.cy * c.g.nx + t.cx);
-    t.its              = intersect_in(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
-    const float rho    = P.density[cnb] * L.scale;
+MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb) {
+    trk&       t  = L.t;
+    const bool bl = c.child < c.nbl;
+    uint64_t   cnb;
+    float      rho;
+    if (bl) {
+        // the beamline child's grid view is rebuilt after stepping rather
+        // than kept live across it (register pressure of the patient path)
+        const grid_view gb = bl_view(P, shb, c.child);
+        const int*     m   = P.bl_meta + 8 * c.child;
+        const uint64_t off = (uint64_t) (uint32_t) m[4] | ((uint64_t) (uint32_t) m[5] << 32);
+        cnb                = (uint64_t) (int32_t) (t.cz * gb.nx * gb.ny + t.cy * gb.nx + t.cx);
+        t.its              = intersect_in(gb, t.p0, t.d0, t.cx, t.cy, t.cz);
+        rho                = P.bl_density[off + cnb];
+    } else {
+        cnb   = (uint64_t) (int32_t) (t.cz * c.g.nx * c.g.ny + t.cy * c.g.nx + t.cx);
+        t.its = intersect_in(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
+        rho   = P.density[cnb] * L.scale;
+    }
     float sp;   // stopping power ratio of the step
     stepping<DBG>(c, t, rho, t.its, sp);
+    if (bl) {
+        const grid_view gb    = bl_view(P, shb, c.child);
+        bool            leave = t.its < 0;
+        if (!leave && !t.stopped) {
+            step_axis(t.p1.x, t.d1.x, gb.xe, 0.0f, t.cx);
+            step_axis(t.p1.y, t.d1.y, gb.ye, 0.0f, t.cy);
+            step_axis(t.p1.z, t.d1.z, gb.ze, 0.0f, t.cz);
+            move(t);
+            leave = !is_valid(gb, t.cx, t.cy, t.cz);
+        }
+        if (t.stopped) {
+            // a stopped track does nothing in the children after this one
+            L.active = false;
+        } else if (leave) {
+            leave_bl(P, c.child, t);
+            ++c.child;
+            L.reenter = true;
+        }
+        if (++L.tsteps > MQI_MAX_TRACK_STEPS && L.active) {
+            L.active = false;
+            atomicOr(P.status, MQI_STATUS_TRACK_CAP);
+        }
+        return;
+    }
     if (t.its < 0) {
         L.active = false;
         return;
@@ -735,6 +876,8 @@ MQI_DI void stage(const transport_params& P, float* sh) {
     for (int i = threadIdx.x; i <= P.nx; i += blockDim.x) xe[i] = P.xe[i];
     for (int i = threadIdx.x; i <= P.ny; i += blockDim.x) ye[i] = P.ye[i];
     for (int i = threadIdx.x; i <= P.nz; i += blockDim.x) ze[i] = P.ze[i];
+    float* blx = ze + P.nz + 1;
+    for (int i = threadIdx.x; i < P.bl_n_edges; i += blockDim.x) blx[i] = P.bl_edges[i];
     __syncthreads();
 }
 
@@ -764,9 +907,13 @@ k_transport(transport_params P) {
     c.cap   = P.stack_cap;
     c.sp    = 0;
     c.drops = 0;
+    c.child = 0;
+    c.nbl   = P.n_bl;
+    c.blxf  = P.bl_xf;
+    const float* shb = c.g.ze + P.nz + 1;   // beamline edges in LDS
     const uint64_t tid      = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t nthreads = (uint64_t) gridDim.x * blockDim.x;
-    c.stack                 = P.stack + tid * (uint64_t) P.stack_cap * 9;
+    c.stack                 = P.stack + tid * (uint64_t) P.stack_cap * MQI_STACK_FLOATS;
 
     // plain: the thread's start_and_length slice; dynamic: the lane's current
     // chunk of P.fetch_chunk consecutive histories from the device counter
@@ -785,6 +932,7 @@ k_transport(transport_params P) {
     L.scen_off = 0;
     L.acc.cnb  = -1;
     L.acc.off  = 0;
+    L.reenter  = false;
     c.g.sx = c.g.sy = c.g.sz = 0.0f;
     unsigned long long done = 0, nstep = 0;
     uint32_t           scen = 0;
@@ -888,15 +1036,22 @@ k_transport(transport_params P) {
             L.nsteps = 0;
             L.tsteps = 0;
             ++done;
-            L.active = enter(P, c, L.t);
-            if (!L.active) continue;
+            c.child   = 0;
+            L.reenter = true;
         } else if (!L.active) {
             pop(c, L.t);
-            L.tsteps = 0;
-            L.active = enter(P, c, L.t);
+            L.tsteps  = 0;
+            c.child   = 0;
+            L.reenter = true;
+        }
+        // a new or popped track, or one that left a beamline child, enters
+        // the next child it reaches (one call site for all three)
+        if (L.reenter) {
+            L.reenter = false;
+            L.active  = enter_chain(P, c, L.t, shb);
             if (!L.active) continue;
         }
-        step<DBG, TRACE>(P, c, L);
+        step<DBG, TRACE>(P, c, L, shb);
         ++nstep;
     }
     flush(P, L.acc);

@@ -1054,7 +1054,8 @@ struct StepLog {
 // transport_particles_patient(_stat) for one history (mqi_transport.hpp:146-249)
 template<typename R, typename Rng, typename M, bool DBG>
 struct Transport {
-    const Grid<R>*            g;
+    const Grid<R>*               g;    // the patient/phantom: the last child, the one with scorers
+    std::vector<const Grid<R>*>  bl;   // beamline children before it (range shifter, aperture)
     Physics<R, Rng, M, DBG>   ph;
     Stack<R>                  stk;
     Scorers*                  sc;
@@ -1133,45 +1134,63 @@ struct Transport {
         }
     }
 
+    // The world's children in order, the patient last (mqi_transport.hpp:158-240):
+    // into the child's frame, to its first voxel, steps until the track
+    // leaves or stops, back to the world frame (also when the child is
+    // missed), then the next child.  Only the patient scores.  Secondaries
+    // are placed with the transform of the child they were made in.
     void run_history(const Vtx<R>& vtx) {
         Track<R> primary;
         primary.v0 = vtx;
         primary.v1 = vtx;
         stk.push_primary(primary);
         while (!stk.empty()) {
-            Track<R> t = stk.pop();
-            // one child (the patient/phantom grid)
-            t.v0.pos = g->inv * (t.v0.pos - g->trans);
-            t.v0.dir = g->inv * (t.v0.dir);
-            t.v0.dir.template normalize<M>();
-            t.v1.pos = t.v0.pos;
-            t.v1.dir = t.v0.dir;
-            int32_t chk[3];
-            g->index(t.v0.pos, t.v0.dir, chk);
-            if (!g->is_valid(chk)) {
-                int32_t c[3];
-                R       dist = g->intersect_out(t.v0.pos, t.v0.dir, c);
-                t.its_dist   = dist;
-                if (dist < 0) continue;
-                t.update_post_vertex_position(dist);
-                t.move();
-                g->index(t.v0.pos, t.v0.dir, t.cell);
-            } else {
-                t.its_dist = 0.0;
-                t.cell[0]  = chk[0];
-                t.cell[1]  = chk[1];
-                t.cell[2]  = chk[2];
-            }
-            while (g->is_valid(t.cell) && !t.stopped) {
-                uint64_t cnb = g->ijk2cnb(t.cell);
-                t.its_dist   = g->intersect_in(t.v0.pos, t.v0.dir, t.cell);
-                ph.stepping(t, stk, (R) g->data[cnb], t.its_dist);
-                if (t.its_dist < 0) break;
-                score(t, cnb);
-                if (!t.stopped) {
-                    g->index_update(t.v1.pos, t.v1.dir, t.cell);
-                    t.move();
+            Track<R> t  = stk.pop();
+            const int nc = (int) bl.size() + 1;
+            for (int k = 0; k < nc; ++k) {
+                const Grid<R>* G       = k < (int) bl.size() ? bl[k] : g;
+                const bool     patient = k == nc - 1;
+                ph.g                   = G;
+                t.v0.pos               = G->inv * (t.v0.pos - G->trans);
+                t.v0.dir               = G->inv * (t.v0.dir);
+                t.v0.dir.template normalize<M>();
+                t.v1.pos = t.v0.pos;
+                t.v1.dir = t.v0.dir;
+                int32_t chk[3];
+                G->index(t.v0.pos, t.v0.dir, chk);
+                bool inside = true;
+                if (!G->is_valid(chk)) {
+                    int32_t c[3];
+                    R       dist = G->intersect_out(t.v0.pos, t.v0.dir, c);
+                    t.its_dist   = dist;
+                    if (dist < 0) {
+                        inside = false;
+                    } else {
+                        t.update_post_vertex_position(dist);
+                        t.move();
+                        G->index(t.v0.pos, t.v0.dir, t.cell);
+                    }
+                } else {
+                    t.its_dist = 0.0;
+                    t.cell[0]  = chk[0];
+                    t.cell[1]  = chk[1];
+                    t.cell[2]  = chk[2];
                 }
+                while (inside && G->is_valid(t.cell) && !t.stopped) {
+                    uint64_t cnb = G->ijk2cnb(t.cell);
+                    t.its_dist   = G->intersect_in(t.v0.pos, t.v0.dir, t.cell);
+                    ph.stepping(t, stk, (R) G->data[cnb], t.its_dist);
+                    if (t.its_dist < 0) break;
+                    if (patient) score(t, cnb);
+                    if (!t.stopped) {
+                        G->index_update(t.v1.pos, t.v1.dir, t.cell);
+                        t.move();
+                    }
+                }
+                t.v0.pos = G->fwd * (t.v0.pos) + G->trans;
+                t.v0.dir = G->fwd * (t.v0.dir);
+                t.v1.pos = t.v0.pos;
+                t.v1.dir = t.v0.dir;
             }
         }
     }
@@ -1272,11 +1291,32 @@ struct Job {
     StepLog*        log;
 };
 
+// Beamline children set by oracle_set_beamline (test set-up: one world at a
+// time): dims, edges, densities and the fwd rotation + translation of each.
+struct BeamlineDesc {
+    std::vector<int>                dims;    // 3 per child
+    std::vector<std::vector<float>> xe, ye, ze, dens;
+    std::vector<float>              xf;      // 12 per child: fwd[9], translation[3]
+};
+BeamlineDesc g_beamline;
+
 template<typename R, typename Rng, typename M, bool DBG>
 void
 run_job(const Job& J) {
     Grid<R> g;
     g.template init<M>(J.nx, J.ny, J.nz, J.xe, J.ye, J.ze, J.density);
+    const int              nbl = (int) g_beamline.dims.size() / 3;
+    std::vector<Grid<R>>   bgrids(nbl);
+    for (int k = 0; k < nbl; ++k) {
+        Grid<R>& b = bgrids[k];
+        b.template init<M>(g_beamline.dims[3 * k], g_beamline.dims[3 * k + 1], g_beamline.dims[3 * k + 2],
+                           g_beamline.xe[k].data(), g_beamline.ye[k].data(), g_beamline.ze[k].data(),
+                           g_beamline.dens[k].data());
+        const float* f = g_beamline.xf.data() + 12 * k;
+        b.fwd          = M3<R>(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8]);
+        b.inv          = b.fwd.inverse();
+        b.trans        = V3<R>(f[9], f[10], f[11]);
+    }
     if (J.rot9) {
         g.fwd = M3<R>(J.rot9[0], J.rot9[1], J.rot9[2], J.rot9[3], J.rot9[4], J.rot9[5], J.rot9[6],
                       J.rot9[7], J.rot9[8]);
@@ -1313,6 +1353,7 @@ run_job(const Job& J) {
         scs[t].init(J.mask, nvox);
         Rng rng;
         Transport<R, Rng, M, DBG> tr(&g, &rng, &scs[t], J.kernel, J.log);
+        for (auto& b : bgrids) tr.bl.push_back(&b);
         int64_t q = total / nt, rmd = total % nt;
         int64_t h0 = q * t + (t < rmd ? t : rmd);
         int64_t h1 = h0 + q + (t < rmd ? 1 : 0);
@@ -1415,6 +1456,29 @@ physics_probe(const float* ek, int64_t n, float rho, float* out) {
 
 
 extern "C" {
+
+// Beamline children of the next transports (n = 0: none): dims 3 per child,
+// edges concatenated (nx+1, ny+1, nz+1 per child), densities concatenated,
+// xf 12 per child (fwd rotation row-major, translation).
+void
+oracle_set_beamline(int n, const int* dims, const float* edges, const float* dens, const float* xf) {
+    g_beamline = BeamlineDesc{};
+    size_t eo = 0, dofs = 0;
+    for (int k = 0; k < n; ++k) {
+        const int nx = dims[3 * k], ny = dims[3 * k + 1], nz = dims[3 * k + 2];
+        g_beamline.dims.insert(g_beamline.dims.end(), {nx, ny, nz});
+        g_beamline.xe.emplace_back(edges + eo, edges + eo + nx + 1);
+        eo += nx + 1;
+        g_beamline.ye.emplace_back(edges + eo, edges + eo + ny + 1);
+        eo += ny + 1;
+        g_beamline.ze.emplace_back(edges + eo, edges + eo + nz + 1);
+        eo += nz + 1;
+        const size_t nv = (size_t) nx * ny * nz;
+        g_beamline.dens.emplace_back(dens + dofs, dens + dofs + nv);
+        dofs += nv;
+        g_beamline.xf.insert(g_beamline.xf.end(), xf + 12 * k, xf + 12 * k + 12);
+    }
+}
 
 int
 oracle_transport(int prec, int rng, int math, int debug, uint64_t master_seed, int nx, int ny, int nz,

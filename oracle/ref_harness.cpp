@@ -107,10 +107,20 @@ log_hit(const mqi::track_t<R>& trk, const mqi::cnb_t& cnb, mqi::grid3d<mqi::dens
     return 0.0;
 }
 
+// Beamline children set by ref_set_beamline (test set-up, one world at a
+// time): the reference's own node_t/grid3d, no scorers, placed before the
+// phantom as tps_env::setup_world does (mqi_tps_env.hpp:729-753).
+struct beamline_child {
+    std::vector<float>              xe, ye, ze, dens;
+    mqi::grid3d<mqi::density_t, R>* geo = nullptr;
+    mqi::node_t<R>                  node;
+};
+std::vector<beamline_child*> g_beamline;
+
 struct world_t {
     mqi::node_t<R>                     world;
     mqi::node_t<R>                     child;
-    mqi::node_t<R>*                    children[1];
+    std::vector<mqi::node_t<R>*>       children;
     mqi::grid3d<mqi::density_t, R>*    geo = nullptr;
     std::vector<mqi::scorer<R>*>       scorers;
     std::vector<mqi::key_value*>       tables;
@@ -175,11 +185,13 @@ build_world(world_t& w, mqi::grid3d<mqi::density_t, R>* geo, int mask) {
     w.child.n_scorers  = (uint16_t) w.scorers.size();
     w.child.scorers    = w.scorers.data();
     w.child.n_children = 0;
-    w.children[0]      = &w.child;
+    w.children.clear();
+    for (beamline_child* b : g_beamline) w.children.push_back(&b->node);
+    w.children.push_back(&w.child);
     w.world.geo        = nullptr;
     w.world.n_scorers  = 0;
-    w.world.n_children = 1;
-    w.world.children   = w.children;
+    w.world.n_children = (uint16_t) w.children.size();
+    w.world.children   = w.children.data();
 }
 
 void
@@ -190,6 +202,36 @@ free_world(world_t& w) {
     }
     for (auto* t : w.tables) delete[] t;
     for (auto* r : w.rois) delete r;
+}
+
+extern "C" void
+ref_set_beamline(int n, const int* dims, const float* edges, const float* dens, const float* xf) {
+    for (beamline_child* b : g_beamline) {
+        delete b->geo;
+        delete b;
+    }
+    g_beamline.clear();
+    size_t eo = 0, dofs = 0;
+    for (int k = 0; k < n; ++k) {
+        auto*     b  = new beamline_child;
+        const int nx = dims[3 * k], ny = dims[3 * k + 1], nz = dims[3 * k + 2];
+        b->xe.assign(edges + eo, edges + eo + nx + 1);
+        eo += nx + 1;
+        b->ye.assign(edges + eo, edges + eo + ny + 1);
+        eo += ny + 1;
+        b->ze.assign(edges + eo, edges + eo + nz + 1);
+        eo += nz + 1;
+        b->dens.assign(dens + dofs, dens + dofs + (size_t) nx * ny * nz);
+        dofs += (size_t) nx * ny * nz;
+        const float* f = xf + 12 * k;
+        b->geo         = make_grid(nx, ny, nz, b->xe.data(), b->ye.data(), b->ze.data(), b->dens.data(), f, f + 9);
+        b->node.geo        = b->geo;
+        b->node.n_scorers  = 0;
+        b->node.scorers    = nullptr;
+        b->node.n_children = 0;
+        b->node.children   = nullptr;
+        g_beamline.push_back(b);
+    }
 }
 
 mqi::patient_material_t<R>*

@@ -22,6 +22,7 @@ def _session_ctx():
 
 @pytest.fixture
 def ctx(_session_ctx):
-    """The session's context with the nominal scenario only."""
+    """The session's context with the nominal scenario only and no beamline."""
     _session_ctx.set_scenarios()
+    _session_ctx.set_beamline()
     return _session_ctx

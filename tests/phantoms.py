@@ -27,3 +27,27 @@ def spots(energies, n, sigma=3.0, sigma_e_frac=0.0, positions=None):
         pos = (0.0, 0.0, 0.5) if positions is None else positions[k]
         out.append(rb.make_spot(e, sigma, n, pos=pos, sigma_e=e * sigma_e_frac, energy_const=(sigma_e_frac == 0.0)))
     return np.array(out, dtype=rb.SPOT_DTYPE)
+
+
+def range_shifter(z0=20.0, z1=40.0, half=30.0, rho=1.19e-3):
+    """One-voxel box of uniform density, as tps_env::create_rangeshifter
+    builds it (grid3d(min, max, 2) per axis, mqi_tps_env.hpp:1606-1650)."""
+    return rb.Grid(rb.uniform_edges(-half, half, 1), rb.uniform_edges(-half, half, 1), rb.uniform_edges(z0, z1, 1),
+                   np.full(1, rho, np.float32))
+
+
+def aperture(radius=8.0, half=15.0, thick=10.0, centre=(0.0, 0.0, 10.0), rot_deg=90.0):
+    """1 mm voxels, density 1e-8 in a circular opening and 100 elsewhere, as
+    tps_env::create_voxelized_aperture fills them (mqi_tps_env.hpp:1652-1736),
+    in a local frame rotated about z and translated to `centre`."""
+    n, nz = int(2 * half), int(thick)
+    xe = np.float32(-half) + np.arange(n + 1, dtype=np.float32)
+    ze = np.float32(-thick / 2) + np.arange(nz + 1, dtype=np.float32)
+    xc = xe[:-1] + np.float32(0.5)
+    inside = (xc[None, :] ** 2 + xc[:, None] ** 2) < radius * radius   # [y, x]
+    plane = np.where(inside, np.float32(1e-8), np.float32(100.0)).astype(np.float32)
+    dens = np.ascontiguousarray(np.broadcast_to(plane, (nz, n, n)).reshape(-1), np.float32)
+    a = np.deg2rad(rot_deg)
+    c, s = np.float32(np.cos(a)), np.float32(np.sin(a))
+    rot = np.array([c, -s, 0, s, c, 0, 0, 0, 1], np.float32)
+    return rb.Grid(xe, xe.copy(), ze, dens, rot9=rot, t3=np.array(centre, np.float32))

@@ -236,3 +236,30 @@ def coordinate_rotation(angles, math=0):
     a = np.ascontiguousarray(angles, np.float32)
     oracle().oracle_coordinate_rotation(math, ptr(a, c_f), ptr(out, c_f))
     return out
+
+
+def _beamline_args(children):
+    """(n, dims int32[3n], edges float32, densities float32, xf float32[12n])
+    from Grid objects (rot9/t3 default to identity / zero)."""
+    dims, edges, dens, xf = [], [], [], []
+    for g in children:
+        dims += [g.nx, g.ny, g.nz]
+        edges += [g.xe, g.ye, g.ze]
+        dens.append(g.density)
+        r = g.rot9 if g.rot9 is not None else np.eye(3, dtype=np.float32).ravel()
+        t = g.t3 if g.t3 is not None else np.zeros(3, np.float32)
+        xf += [np.asarray(r, np.float32).ravel(), np.asarray(t, np.float32).ravel()]
+    cat = lambda a, dt: np.ascontiguousarray(np.concatenate(a) if a else np.zeros(1, dt), dt)  # noqa: E731
+    return (len(children), np.ascontiguousarray(dims or [0], np.int32), cat(edges, np.float32),
+            cat(dens, np.float32), cat(xf, np.float32))
+
+
+def set_beamline(children):
+    """Beamline children (before the phantom) of the oracle's and, where it
+    is built, the reference harness's next transports; [] clears them."""
+    n, dims, edges, dens, xf = _beamline_args(children)
+    args = (n, ptr(dims, ctypes.POINTER(ctypes.c_int)), ptr(edges, c_f), ptr(dens, c_f), ptr(xf, c_f))
+    oracle().oracle_set_beamline(*args)
+    if have_reference():
+        for debug in (0, 1):
+            reference(debug).ref_set_beamline(*args)

@@ -60,6 +60,8 @@ int main(void) {
     F(mqi_run_config, stat_threshold) F(mqi_run_config, max_rounds) F(mqi_run_config, dij)
     F(mqi_run_config, dij_capacity) F(mqi_run_config, hist_offset) F(mqi_run_config, dij_slots_sqrt)
     F(mqi_run_config, use_graph)
+    printf("mqi_grid_desc %zu\n", sizeof(mqi_grid_desc));
+    F(mqi_grid_desc, xe) F(mqi_grid_desc, density) F(mqi_grid_desc, rotation) F(mqi_grid_desc, translation)
     F(mqi_run_result, dropped) F(mqi_run_result, steps) F(mqi_run_result, status) F(mqi_run_result, kernel_ms)
     F(mqi_run_result, stat)
     return 0;
@@ -85,6 +87,9 @@ def test_ctypes_mirrors_match_c_layout():
     assert lay["mqi_run_result"] == ctypes.sizeof(_abi.RunResult)
     for f in ("stat_threshold", "max_rounds", "dij", "dij_capacity", "hist_offset", "dij_slots_sqrt", "use_graph"):
         assert lay["mqi_run_config." + f] == getattr(_abi.RunConfig, f).offset, f
+    assert lay["mqi_grid_desc"] == ctypes.sizeof(_abi.GridDesc)
+    for f in ("xe", "density", "rotation", "translation"):
+        assert lay["mqi_grid_desc." + f] == getattr(_abi.GridDesc, f).offset, f
     for f in ("dropped", "steps", "status", "kernel_ms", "stat"):
         assert lay["mqi_run_result." + f] == getattr(_abi.RunResult, f).offset, f
 

@@ -206,3 +206,41 @@ def test_cpp_phantom_driver_matches_abi(ctx, tmp_path):
     assert res.histories == 600
     assert dose_cpp.shape == ctx.scorer(0).shape
     assert np.allclose(dose_cpp, ctx.scorer(0), rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("debug", [1, 0])
+def test_exact_beamline_children_match_oracle(ctx, debug):
+    """A range shifter and a rotated, translated voxelised aperture before the
+    slab phantom: the verification kernels give the oracle's dose (which
+    equals the reference's, tests/test_oracle_vs_reference.py), with the
+    secondaries made in the range shifter placed by its transform."""
+    g = phantoms.slab_box(24, 24, 160)
+    sp = phantoms.spots([150.0, 130.0], 300, sigma=6.0, sigma_e_frac=0.01,
+                        positions=[(0, 0, 50.0), (3, -2, 50.0)])
+    bl = [phantoms.range_shifter(), phantoms.aperture()]
+    rb.set_beamline(bl)
+    try:
+        ref, ntr = rb.oracle_run_spots(g, sp, mask=rb.SC_DOSE | rb.SC_LETD, kernel=2, prec=32, rng=1, math=1,
+                                       debug=debug, master_seed=17, threads=4)
+    finally:
+        rb.set_beamline([])
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_beamline(bl)
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE | rb.SC_LETD)
+    res = ctx.run(ctx.config(seed=17, exact=True, batched=True, physics_debug=bool(debug)))
+    assert res.histories == ntr == 600
+    for k in range(3):
+        got, want = ctx.scorer(k), ref[k]
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got), fin)
+        assert np.allclose(got[fin], want[fin], rtol=1e-9, atol=0), k
+    # the production kernels through the same beamline: same physics, other
+    # arithmetic; the aperture still cuts the off-axis dose
+    ctx.zero_scorers()
+    ctx.run(ctx.config(seed=17, batched=True, physics_debug=bool(debug)))
+    d = ctx.scorer(0).reshape(160, 24, 24).sum(0)
+    xc = np.arange(24) - 11.5
+    r2 = xc[None, :] ** 2 + xc[:, None] ** 2
+    assert d[r2 > 100].sum() < 0.2 * d.sum()
+    ctx.set_beamline()

@@ -47,6 +47,33 @@ def test_slab_transport_bit_exact(debug):
 
 
 @needs_ref
+@pytest.mark.parametrize("debug", [1, 0])
+def test_beamline_children_bit_exact(debug):
+    """A range shifter and a rotated, translated voxelised aperture before a
+    slab phantom (the world of tps_env::setup_world with beamline objects):
+    the child loop, the frame round trips between children, the aperture's
+    stop density and the secondaries made in the range shifter agree bit for
+    bit."""
+    g = phantoms.slab_box(24, 24, 160)
+    sp = rb.make_spot(150.0, 6.0, 200, pos=(0, 0, 50.0), energy_const=False, sigma_e=1.5)
+    vtx, seeds = rb.ref_sample_spot(sp, [0, 0, 0, 0], [0, 0, 0], 11, 2, 200, debug=debug)
+    rb.set_beamline([phantoms.range_shifter(), phantoms.aperture()])
+    try:
+        for mask, kernel in [(rb.SC_DOSE | rb.SC_LETD, 0), (rb.SC_DOSE | rb.SC_STAT, 1)]:
+            ra, nr = rb.ref_transport(debug, g, vtx, seeds, mask=mask, kernel=kernel)
+            oa, no = rb.oracle_transport(g, vtx, seeds, mask=mask, kernel=kernel, debug=debug)
+            assert nr == no == 200
+            for a, b in zip(ra, oa):
+                assert np.array_equal(_bits(a), _bits(b))
+        # the aperture blocks: less dose than without the beamline, none far off axis
+        rb.set_beamline([])
+        free, _ = rb.oracle_transport(g, vtx, seeds, mask=rb.SC_DOSE, kernel=0, debug=debug)
+        assert oa[0].sum() < free[0].sum()
+    finally:
+        rb.set_beamline([])
+
+
+@needs_ref
 def test_rotated_translated_grid_bit_exact():
     """Non-identity grid rotation and translation (daughter placement quirk included)."""
     g = phantoms.water_box(20, 20, 80)
