@@ -123,8 +123,11 @@ int mqi_set_spots(mqi_ctx* ctx, const mqi_spot* spots, int n_spots);
 int mqi_set_scenarios(mqi_ctx* ctx, int n, const float* shift_xyz, const float* density_scale, const float* budget);
 
 /* Scorers (MQI_SCORE_* bits of mqi_hip_types.h), in the reference's order
- * Dose, LETd numerator, LETd denominator, Dose_stat, DoseSquare_stat
- * (mqi_tps_env.hpp:788-960); per_scenario != 0 keeps one array per scenario.
+ * Dose, LETd numerator, LETd denominator, LETt numerator, LETt denominator,
+ * energy deposit, dose to medium, Dose_stat, DoseSquare_stat
+ * (mqi_tps_env.hpp:788-960; the statistics pair last, as the _stat kernel
+ * expects); at most 6 arrays (5 with cfg.dij); per_scenario != 0 keeps one
+ * array per scenario.
  * Clears them.  Replaces add_node_scorers (mqi_upload_data.hpp:106-142). */
 int mqi_set_scorers(mqi_ctx* ctx, int mask, int per_scenario);
 int mqi_zero_scorers(mqi_ctx* ctx);

@@ -62,7 +62,10 @@ enum {
 enum {
     MQI_SCORE_DOSE = 1,
     MQI_SCORE_LETD = 2,
-    MQI_SCORE_STAT = 4
+    MQI_SCORE_STAT = 4,
+    MQI_SCORE_LETT = 16,   /* LETt numerator and denominator (LETt_weight1/2, :141-177) */
+    MQI_SCORE_EDEP = 32,   /* energy deposit dE + local dE, MeV (energy_deposit, :14-20) */
+    MQI_SCORE_DM   = 64    /* dose to medium of primaries, Gy (dose_to_medium, :80-92) */
 };
 
 #ifdef __cplusplus

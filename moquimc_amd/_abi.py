@@ -17,7 +17,7 @@ SPOT_DTYPE = np.dtype([
     ("histories", np.uint64)])
 assert SPOT_DTYPE.itemsize == 128
 
-SCORE_DOSE, SCORE_LETD, SCORE_STAT = 1, 2, 4
+SCORE_DOSE, SCORE_LETD, SCORE_STAT, SCORE_LETT, SCORE_EDEP, SCORE_DM = 1, 2, 4, 16, 32, 64
 LOOPS_REFERENCE_PLAIN, LOOPS_REFERENCE_STAT, LOOPS_ONCE = 0, 1, 2
 
 

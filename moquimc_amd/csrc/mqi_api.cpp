@@ -848,7 +848,14 @@ int
 mqi_set_scorers(mqi_ctx* c, int mask, int per_scenario) {
     int r = ready(c);
     if (r) return r;
-    if (mask & ~(MQI_SCORE_DOSE | MQI_SCORE_LETD | MQI_SCORE_STAT)) return fail(c, MQI_E_ARG, "unknown scorer bits");
+    if (mask & ~(MQI_SCORE_DOSE | MQI_SCORE_LETD | MQI_SCORE_STAT | MQI_SCORE_LETT | MQI_SCORE_EDEP | MQI_SCORE_DM))
+        return fail(c, MQI_E_ARG, "unknown scorer bits");
+    {
+        const int n = ((mask & MQI_SCORE_DOSE) ? 1 : 0) + ((mask & MQI_SCORE_LETD) ? 2 : 0) +
+                      ((mask & MQI_SCORE_LETT) ? 2 : 0) + ((mask & MQI_SCORE_EDEP) ? 1 : 0) +
+                      ((mask & MQI_SCORE_DM) ? 1 : 0) + ((mask & MQI_SCORE_STAT) ? 2 : 0);
+        if (n > MQI_MAX_SCORERS) return fail(c, MQI_E_ARG, "%d scorer arrays requested, at most %d", n, MQI_MAX_SCORERS);
+    }
     c->mask     = mask;
     c->per_scen = per_scenario ? 1 : 0;
     c->kinds.clear();
@@ -857,6 +864,12 @@ mqi_set_scorers(mqi_ctx* c, int mask, int per_scenario) {
         c->kinds.push_back(MQI_KIND_LETD_NUM);
         c->kinds.push_back(MQI_KIND_LETD_DEN);
     }
+    if (mask & MQI_SCORE_LETT) {
+        c->kinds.push_back(MQI_KIND_LETT_NUM);
+        c->kinds.push_back(MQI_KIND_LETT_DEN);
+    }
+    if (mask & MQI_SCORE_EDEP) c->kinds.push_back(MQI_KIND_EDEP);
+    if (mask & MQI_SCORE_DM) c->kinds.push_back(MQI_KIND_DM);
     if (mask & MQI_SCORE_STAT) {
         c->kinds.push_back(MQI_KIND_STAT_DOSE);
         c->kinds.push_back(MQI_KIND_STAT_DOSE2);
@@ -899,6 +912,7 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     if (cfg.scoring_loops < 0 || cfg.scoring_loops > 2) return fail(c, MQI_E_ARG, "bad scoring_loops");
     const int n_sc = (int) c->kinds.size() + (cfg.dij ? 1 : 0);
     if (n_sc == 0) return fail(c, MQI_E_STATE, "no scorers");
+    if (n_sc > MQI_MAX_SCORERS) return fail(c, MQI_E_ARG, "the Dij scorer needs one of the %d scorer slots", MQI_MAX_SCORERS);
     if (cfg.dij && (r = alloc_dij(c, cfg))) return r;
     KCHK(c, mqi_aux::fill_u64(c->d_counters, 0ull, 4, c->stream));
     KCHK(c, mqi_aux::fill_u32(c->d_status, 0u, 1, c->stream));

@@ -418,6 +418,7 @@ struct trk {
     float dE, ldE, its;
     int   cx, cy, cz;
     bool  stopped;
+    bool  primary;   // the history's own track (daughters are not)
 };
 
 MQI_DI void update_pos(trk& t, float len) {   // mqi_track.hpp:178-180

@@ -17,7 +17,11 @@ enum {
     MQI_KIND_LETD_DEN    = 2,   // LETd_weight2
     MQI_KIND_STAT_DOSE   = 3,   // Dose_stat (dose_to_water)
     MQI_KIND_STAT_DOSE2  = 4,   // DoseSquare_stat (dose_to_water_square)
-    MQI_KIND_DIJ         = 5    // dose_to_water keyed by (voxel, spot)
+    MQI_KIND_DIJ         = 5,   // dose_to_water keyed by (voxel, spot)
+    MQI_KIND_LETT_NUM    = 6,   // LETt_weight1
+    MQI_KIND_LETT_DEN    = 7,   // LETt_weight2
+    MQI_KIND_EDEP        = 8,   // energy_deposit
+    MQI_KIND_DM          = 9    // dose_to_medium (primaries)
 };
 
 // scoring loop semantics (MQI_LOOPS_* of mqi_hip.h)

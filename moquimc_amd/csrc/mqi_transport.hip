@@ -432,6 +432,24 @@ MQI_DI double dose_hit(const trk& t, float density, float volume, float sp) {
     return (double) ((t.dE + t.ldE) * 1.60218e-10f / (volume * density * sp));
 #endif
 }
+// LETt_weight1 / LETt_weight2 (mqi_scorer_energy_deposit.hpp:141-177)
+MQI_DI double lett_hit(const trk& t, float density, bool numer) {
+    density *= 1000.0;
+    double length = (t.p1.x - t.p0.x) * (t.p1.x - t.p0.x);
+    length += (t.p1.y - t.p0.y) * (t.p1.y - t.p0.y);
+    length += (t.p1.z - t.p0.z) * (t.p1.z - t.p0.z);
+    length = MQ_SQRTD(length);
+    if (length <= 0) return 0.0;
+    if (!numer) return length;
+    double let = t.dE / length / density;
+    return length * let;
+}
+// energy_deposit (:14-20); dose_to_medium of primaries in J/kg (:80-92)
+MQI_DI double edep_hit(const trk& t) { return t.dE + t.ldE; }
+MQI_DI double dm_hit(const trk& t, float density, float volume) {
+    return t.primary ? t.dE * 1.60218e-13 * 1000.0 / (volume * density) : 0.0;
+}
+
 MQI_DI double letd_hit(const trk& t, float density, bool numer) {
     density *= 1000.0;
     double length = (t.p1.x - t.p0.x) * (t.p1.x - t.p0.x);
@@ -511,6 +529,14 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             v[s] = letd_hit(t, density, true);
         else if (k == MQI_KIND_LETD_DEN)
             v[s] = letd_hit(t, density, false);
+        else if (k == MQI_KIND_LETT_NUM)
+            v[s] = lett_hit(t, density, true);
+        else if (k == MQI_KIND_LETT_DEN)
+            v[s] = lett_hit(t, density, false);
+        else if (k == MQI_KIND_EDEP)
+            v[s] = edep_hit(t);
+        else if (k == MQI_KIND_DM)
+            v[s] = dm_hit(t, density, volume);
         else
             v[s] = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
     }
@@ -695,6 +721,7 @@ MQI_DI void pop(ctx& c, trk& t) {
     t.dE           = s[8];
     t.ldE          = 0.0f;
     t.stopped      = false;
+    t.primary      = false;
 }
 
 // Vertex of history h of item (scenario, spot): phsp_6d + norm_1d/const_1d
@@ -741,6 +768,7 @@ MQI_DI void sample_vertex(const transport_params& P, const mqi_spot& sp, uint32_
     t.dE   = 0;
     t.ldE  = 0;
     t.stopped = false;
+    t.primary = true;
 }
 
 // history g of the flattened work list -> (item, history-in-item)
@@ -1029,6 +1057,7 @@ k_transport(transport_params P) {
                 L.t.dE         = 0;
                 L.t.ldE        = 0;
                 L.t.stopped    = false;
+                L.t.primary    = true;
             } else {
                 sample_vertex(P, P.spots[spot_id], spot_id, scen, L.hist + hoff, L.t);
             }

@@ -1022,11 +1022,43 @@ letd_hit(const Track<R>& trk, uint64_t cnb, const Grid<R>& g, bool numer) {
     return numer ? trk.dE * let : trk.dE * 1.0;
 }
 
-enum { SC_DOSE = 1, SC_LETD = 2, SC_STAT = 4, SC_LOG = 8 };
+// LETt_weight1 / LETt_weight2 (mqi_scorer_energy_deposit.hpp:141-177)
+template<typename R, typename M>
+double
+lett_hit(const Track<R>& trk, uint64_t cnb, const Grid<R>& g, bool numer) {
+    R density = g.data[cnb];
+    density *= 1000.0;
+    double length = (trk.v1.pos.x - trk.v0.pos.x) * (trk.v1.pos.x - trk.v0.pos.x);
+    length += (trk.v1.pos.y - trk.v0.pos.y) * (trk.v1.pos.y - trk.v0.pos.y);
+    length += (trk.v1.pos.z - trk.v0.pos.z) * (trk.v1.pos.z - trk.v0.pos.z);
+    length = M::sq(length);
+    if (length <= 0) return 0.0;
+    if (!numer) return length;
+    double let = trk.dE / length / density;
+    return length * let;
+}
+// energy_deposit (:14-20) and dose_to_medium (:80-92)
+template<typename R>
+double
+edep_hit(const Track<R>& trk) {
+    return trk.dE + trk.local_dE;
+}
+template<typename R>
+double
+dm_hit(const Track<R>& trk, uint64_t cnb, const Grid<R>& g) {
+    R density = g.data[cnb];
+    R volume  = g.volume(cnb);
+    return trk.primary ? trk.dE * 1.60218e-13 * 1000.0 / (volume * density) : 0.0;
+}
+
+enum { SC_DOSE = 1, SC_LETD = 2, SC_STAT = 4, SC_LOG = 8, SC_LETT = 16, SC_EDEP = 32, SC_DM = 64 };
 
 struct Scorers {
     int                               mask = 0;
-    std::vector<int>                  kinds;   // 0 dose, 1 letd num, 2 letd den, 3 stat dose, 4 stat dose^2
+    // 0 dose, 1 letd num, 2 letd den, 3 stat dose, 4 stat dose^2, 5 lett num,
+    // 6 lett den, 7 energy deposit, 8 dose to medium; laid out as tps_env
+    // does: dose, letd, lett, edep, dm, the statistics pair last
+    std::vector<int>                  kinds;
     std::vector<std::vector<double>>  data;
     void init(int m, uint64_t nvox) {
         mask = m;
@@ -1036,6 +1068,12 @@ struct Scorers {
             kinds.push_back(1);
             kinds.push_back(2);
         }
+        if (m & SC_LETT) {
+            kinds.push_back(5);
+            kinds.push_back(6);
+        }
+        if (m & SC_EDEP) kinds.push_back(7);
+        if (m & SC_DM) kinds.push_back(8);
         if (m & SC_STAT) {
             kinds.push_back(3);
             kinds.push_back(4);
@@ -1075,6 +1113,10 @@ struct Transport {
         case 3: return dose_hit<R, M, DBG>(t, cnb, *g);
         case 1: return letd_hit<R, M>(t, cnb, *g, true);
         case 2: return letd_hit<R, M>(t, cnb, *g, false);
+        case 5: return lett_hit<R, M>(t, cnb, *g, true);
+        case 6: return lett_hit<R, M>(t, cnb, *g, false);
+        case 7: return edep_hit<R>(t);
+        case 8: return dm_hit<R>(t, cnb, *g);
         default: {
             double d = dose_hit<R, M, DBG>(t, cnb, *g);
             return d * d;
@@ -1114,7 +1156,7 @@ struct Transport {
             // stat scorers through get_mask_idx(cnb) >= 0 (what the _stat
             // kernel does; for fewer than three scorers also the plain kernel)
             for (int s = 0; s < n; ++s) {
-                const bool stat = sc->kinds[s] >= 3;
+                const bool stat = sc->kinds[s] == 3 || sc->kinds[s] == 4;
                 if (stat ? roi >= 0 : roi > 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
             }
             return;
@@ -1427,7 +1469,8 @@ dispatch_prec(const Job& J) {
 
 int
 n_scorers(int mask) {
-    return ((mask & SC_DOSE) ? 1 : 0) + ((mask & SC_LETD) ? 2 : 0) + ((mask & SC_STAT) ? 2 : 0);
+    return ((mask & SC_DOSE) ? 1 : 0) + ((mask & SC_LETD) ? 2 : 0) + ((mask & SC_STAT) ? 2 : 0) +
+           ((mask & SC_LETT) ? 2 : 0) + ((mask & SC_EDEP) ? 1 : 0) + ((mask & SC_DM) ? 1 : 0);
 }
 
 }   // namespace orc

@@ -142,7 +142,7 @@ make_grid(int nx, int ny, int nz, const float* xe, const float* ye, const float*
     return g;
 }
 
-enum { SC_DOSE = 1, SC_LETD = 2, SC_STAT = 4, SC_LOG = 8 };
+enum { SC_DOSE = 1, SC_LETD = 2, SC_STAT = 4, SC_LOG = 8, SC_LETT = 16, SC_EDEP = 32, SC_DM = 64 };
 
 void
 add_scorer(world_t& w, const char* name, uint32_t nvox, mqi::fp_compute_hit<R> fp, mqi::roi_t* roi) {
@@ -176,6 +176,12 @@ build_world(world_t& w, mqi::grid3d<mqi::density_t, R>* geo, int mask) {
             add_scorer(w, "LETd_numer", nvox, mqi::LETd_weight1<R>, direct);
             add_scorer(w, "LETd_denom", nvox, mqi::LETd_weight2<R>, direct);
         }
+        if (mask & SC_LETT) {
+            add_scorer(w, "LETt_numer", nvox, mqi::LETt_weight1<R>, direct);
+            add_scorer(w, "LETt_denom", nvox, mqi::LETt_weight2<R>, direct);
+        }
+        if (mask & SC_EDEP) add_scorer(w, "Edep", nvox, mqi::energy_deposit<R>, direct);
+        if (mask & SC_DM) add_scorer(w, "Dm", nvox, mqi::dose_to_medium<R>, direct);
         if (mask & SC_STAT) {
             add_scorer(w, "Dose_stat", nvox, mqi::dose_to_water<R>, direct);
             add_scorer(w, "DoseSquare_stat", nvox, mqi::dose_to_water_square<R>, direct);

@@ -27,7 +27,7 @@ SPOT_DTYPE = np.dtype([
     ("histories", np.uint64)])
 assert SPOT_DTYPE.itemsize == 128
 
-SC_DOSE, SC_LETD, SC_STAT, SC_LOG = 1, 2, 4, 8
+SC_DOSE, SC_LETD, SC_STAT, SC_LOG, SC_LETT, SC_EDEP, SC_DM = 1, 2, 4, 8, 16, 32, 64
 
 
 def ptr(a, t):
@@ -37,7 +37,8 @@ def ptr(a, t):
 
 
 def n_scorers(mask):
-    return (1 if mask & SC_DOSE else 0) + (2 if mask & SC_LETD else 0) + (2 if mask & SC_STAT else 0)
+    return ((1 if mask & SC_DOSE else 0) + (2 if mask & SC_LETD else 0) + (2 if mask & SC_STAT else 0) +
+            (2 if mask & SC_LETT else 0) + (1 if mask & SC_EDEP else 0) + (1 if mask & SC_DM else 0))
 
 
 _libs = {}

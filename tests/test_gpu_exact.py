@@ -244,3 +244,31 @@ def test_exact_beamline_children_match_oracle(ctx, debug):
     r2 = xc[None, :] ** 2 + xc[:, None] ** 2
     assert d[r2 > 100].sum() < 0.2 * d.sum()
     ctx.set_beamline()
+
+
+@pytest.mark.parametrize("debug", [0, 1])
+def test_exact_lett_edep_dm_match_oracle(ctx, debug):
+    """LETt numerator/denominator, energy deposit and dose to medium (primaries
+    only) from the verification kernels equal the oracle's (which equal the
+    reference's, tests/test_oracle_vs_reference.py) up to FP64 summation order."""
+    g = phantoms.slab_box(30, 30, 200)
+    sp = phantoms.spots([120.0, 170.0], 300, sigma=4.0, sigma_e_frac=0.01, positions=[(0, 0, 0.5), (4, -3, 0.5)])
+    mask = rb.SC_DOSE | rb.SC_LETT | rb.SC_EDEP | rb.SC_DM
+    ref, ntr = rb.oracle_run_spots(g, sp, mask=mask, kernel=2, prec=32, rng=1, math=1, debug=debug, master_seed=23,
+                                   threads=4)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(mask)
+    assert ctx.scorer_count() == 5
+    res = ctx.run(ctx.config(seed=23, exact=True, batched=True, physics_debug=bool(debug)))
+    assert res.histories == ntr == 600
+    for k in range(5):
+        got, want = ctx.scorer(k), ref[k]
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got), fin)
+        assert np.allclose(got[fin], want[fin], rtol=1e-9, atol=0), k
+        assert np.any(want[fin] != 0), k
+    # more than the kernel's six scorer slots is refused
+    with pytest.raises(Exception):
+        ctx.set_scorers(rb.SC_DOSE | rb.SC_LETD | rb.SC_LETT | rb.SC_STAT)
+    ctx.set_scorers(rb.SC_DOSE)

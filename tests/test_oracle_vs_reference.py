@@ -48,6 +48,24 @@ def test_slab_transport_bit_exact(debug):
 
 @needs_ref
 @pytest.mark.parametrize("debug", [1, 0])
+def test_lett_edep_dm_scorers_bit_exact(debug):
+    """LETt numerator and denominator, energy deposit and dose to medium
+    (primaries only), laid out after Dose and LETd and before the statistics
+    pair, through both kernels."""
+    g = phantoms.slab_box(24, 24, 200)
+    vtx, seeds = _spot_vertices(debug, 160.0, 150, sigma_e=1.6)
+    for mask, kernel in [(rb.SC_DOSE | rb.SC_LETT | rb.SC_EDEP | rb.SC_DM, 0),
+                         (rb.SC_LETT | rb.SC_EDEP | rb.SC_STAT, 1)]:
+        ra, nr = rb.ref_transport(debug, g, vtx, seeds, mask=mask, kernel=kernel)
+        oa, no = rb.oracle_transport(g, vtx, seeds, mask=mask, kernel=kernel, debug=debug)
+        assert nr == no == 150 and len(ra) == len(oa) == rb.n_scorers(mask)
+        for a, b in zip(ra, oa):
+            assert np.array_equal(_bits(a), _bits(b))
+            assert np.any(a != 0)
+
+
+@needs_ref
+@pytest.mark.parametrize("debug", [1, 0])
 def test_beamline_children_bit_exact(debug):
     """A range shifter and a rotated, translated voxelised aperture before a
     slab phantom (the world of tps_env::setup_world with beamline objects):
