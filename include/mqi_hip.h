@@ -197,6 +197,24 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
 
+/* Region of interest of the scorers (target 0: Dose, LETd, LETt, Edep, Dm
+ * and Dij) or of the statistics pair (target 1), from a voxel mask (x
+ * fastest, the grid's size; the reference's mask_total, the sum of the
+ * structure masks, mqi_file_handler.hpp:94-113): the CONTOUR mapping that
+ * mask_reader::mask_to_roi builds (mqi_file_handler.hpp:177-217, roi_t
+ * mqi_roi.hpp:18-196).  In voxel order a 1 opens a run and a 0 closes it;
+ * other values continue the current run, so a 0/1 mask scores exactly its
+ * 1 voxels.  A run open at the last voxel ends there.  Scoring goes through
+ * roi_->idx(v) > 0 for the scorers and get_mask_idx(v) >= 0 for the
+ * statistics pair (mqi_transport.hpp:342-364), i.e. a voxel scores when it
+ * is in a run.  mask = NULL restores DIRECT (every voxel, voxel 0 excluded
+ * from the scorers).  Arrays stay dense over the grid: the reference stores
+ * the statistics pair compactly over its ROI, here voxels outside hold 0
+ * and the stopping statistic covers the same voxels.  The runs are expanded
+ * into one byte per voxel on the device (nvox bytes while a mask is set).
+ * Set after mqi_set_grid: setting a grid clears both masks. */
+int mqi_set_roi(mqi_ctx* ctx, int target, const uint8_t* mask);
+
 /* Beamline children transported before the patient grid, in order (the
  * world of tps_env::setup_world, mqi_tps_env.hpp:713-753: range shifter and
  * voxelised aperture nodes, then the phantom; transport loop

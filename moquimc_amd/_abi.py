@@ -89,6 +89,7 @@ def lib():
             "mqi_dij_save_npz": (i, [vp, i, ctypes.c_char_p, d]),
             "mqi_set_grid_ct_mha": (i, [vp, ctypes.c_char_p, f, vp]),
             "mqi_set_beamline": (i, [vp, i, vp]),
+            "mqi_set_roi": (i, [vp, i, vp]),
             "mqi_save_scorer": (i, [vp, i, i, ctypes.c_char_p, ctypes.c_char_p, i, d]),
         }
         for name, (res, args) in sig.items():
@@ -104,7 +105,7 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
-            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline"]
+            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi"]
 
 
 class GridDesc(ctypes.Structure):
@@ -202,6 +203,17 @@ class Context:
             d.translation[:] = list(np.asarray(tr if tr is not None else np.zeros(3), np.float32).ravel())
         self._chk(self._L.mqi_set_beamline(self.h, len(children), ctypes.cast(descs, ctypes.c_void_p)),
                   "mqi_set_beamline")
+
+    def set_roi(self, target, mask=None):
+        """ROI of the scorers (target 0) or of the statistics pair (target 1)
+        from a uint8 voxel mask of the grid's size (x fastest; 1 opens a run,
+        0 closes it, other values continue it); None restores DIRECT."""
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, np.uint8).reshape(-1)
+            if m.size != self.nvox:
+                raise ValueError("mask has %d voxels, the grid %d" % (m.size, self.nvox))
+        self._chk(self._L.mqi_set_roi(self.h, int(target), _p(m)), "mqi_set_roi")
 
     def set_grid_ct_mha(self, path, density_scale=1.0, shift=None):
         """CT from a MetaImage file (float32 HU after the header)."""

@@ -102,6 +102,12 @@ struct mqi_ctx {
     uint32_t* d_item_scen = nullptr;
     size_t    items_cap   = 0;
 
+    // ROIs of the regular scorers ([0]) and the statistics pair ([1]):
+    // CONTOUR runs (start, stride) when set, and their per-voxel expansion
+    bool                  roi_set[2] = {false, false};
+    std::vector<uint32_t> roi_start[2], roi_stride[2];
+    uint8_t*              d_roi_flags = nullptr;
+
     // beamline children (device copies; see transport_params)
     int       bl_n       = 0;
     int       bl_n_edges = 0;
@@ -274,6 +280,7 @@ upload_grid_common(mqi_ctx* c, int nx, int ny, int nz, const float* xe, const fl
     c->nx = nx;
     c->ny = ny;
     c->nz = nz;
+    c->roi_set[0] = c->roi_set[1] = false;   // masks are per grid
     c->h_xe.assign(xe, xe + nx + 1);
     c->h_ye.assign(ye, ye + ny + 1);
     c->h_ze.assign(ze, ze + nz + 1);
@@ -372,6 +379,7 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.status        = c->d_status;
     P.hist_offset   = c->d_hoff;
     P.seed          = cfg.seed;
+    P.roi_flags     = (c->roi_set[0] || c->roi_set[1]) ? c->d_roi_flags : nullptr;
     P.n_bl          = c->bl_n;
     P.bl_meta       = c->d_bl_meta;
     P.bl_xf         = c->d_bl_xf;
@@ -660,6 +668,31 @@ create_failed(int code, const std::string& msg) {
     return code;
 }
 
+// The per-voxel ROI bytes the kernels read: bit 0 where the regular scorers
+// score (inside a run of the scorers' ROI, or v > 0 under DIRECT), bit 1
+// where the statistics pair does (inside a run of its ROI, or everywhere).
+// One byte load per scoring step replaces roi_t's binary search over runs.
+int
+roi_upload(mqi_ctx* c) {
+    const uint64_t       n = nvox(c);
+    std::vector<uint8_t> fl(n, 0);
+    if (!c->roi_set[0])
+        for (uint64_t v = 1; v < n; ++v) fl[v] = 1;
+    if (!c->roi_set[1])
+        for (uint64_t v = 0; v < n; ++v) fl[v] |= 2;
+    for (int w = 0; w < 2; ++w) {
+        if (!c->roi_set[w]) continue;
+        for (size_t r = 0; r < c->roi_start[w].size(); ++r)
+            for (uint64_t v = c->roi_start[w][r]; v < (uint64_t) c->roi_start[w][r] + c->roi_stride[w][r]; ++v)
+                fl[v] |= (uint8_t) (1u << w);
+    }
+    hipSetDevice(c->device);
+    int r = dalloc(c, &c->d_roi_flags, n);
+    if (r) return r;
+    HIPCHK(c, hipMemcpy(c->d_roi_flags, fl.data(), n, hipMemcpyHostToDevice));
+    return 0;
+}
+
 }   // namespace
 
 extern "C" {
@@ -735,6 +768,7 @@ mqi_destroy(mqi_ctx* c) {
     dfree(c->d_dij_keys);
     dfree(c->d_dij_vals);
     dfree(c->d_dij_off);
+    dfree(c->d_roi_flags);
     dfree(c->d_bl_meta);
     dfree(c->d_bl_xf);
     dfree(c->d_bl_edges);
@@ -926,7 +960,7 @@ mqi_run(mqi_ctx* c, const mqi_run_config* cfg_in, mqi_run_result* res) {
     // Dose_stat only and derive Dose at the end of the run
     int       s1 = -1, s2 = -1;
     const bool alias = cfg.scoring_loops == MQI_LOOPS_ONCE && !c->kinds.empty() && c->kinds[0] == MQI_KIND_DOSE &&
-                       stat_index(c, &s1, &s2) == 0;
+                       stat_index(c, &s1, &s2) == 0 && !c->roi_set[0] && !c->roi_set[1];
     const uint64_t n_all = nvox(c) * (c->per_scen ? (uint64_t) c->n_scen : 1ull);
     if (alias) {
         if (c->snap_n != n_all) {
@@ -1347,6 +1381,37 @@ mqi_dij_copy(mqi_ctx* c, int s, int64_t* indptr, int32_t* indices, float* values
     if (indices && m.nnz) HIPCHK(c, hipMemcpy(indices, m.col, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
     if (values && m.nnz) HIPCHK(c, hipMemcpy(values, m.val, sizeof(float) * m.nnz, hipMemcpyDeviceToHost));
     return 0;
+}
+
+int
+mqi_set_roi(mqi_ctx* c, int target, const uint8_t* mask) {
+    if (!c) return MQI_E_ARG;
+    if (target != 0 && target != 1) return fail(c, MQI_E_ARG, "roi target %d (0 scorers, 1 statistics)", target);
+    c->roi_set[target] = false;
+    if (!mask) return c->roi_set[1 - target] ? roi_upload(c) : 0;
+    int r = ready(c);
+    if (r) return r;
+    // mask_reader::mask_to_roi (mqi_file_handler.hpp:177-217): in voxel order
+    // a 1 opens a run and a 0 closes it; other values (overlapping structures
+    // summed) neither open nor close one.  A run still open at the last voxel
+    // ends there (the reference leaves that stride unset).
+    std::vector<uint32_t> start, stride;
+    const uint64_t        n    = nvox(c);
+    bool                  open = false;
+    for (uint64_t v = 0; v < n; ++v) {
+        if (mask[v] == 1 && !open) {
+            open = true;
+            start.push_back((uint32_t) v);
+        } else if (mask[v] == 0 && open) {
+            open = false;
+            stride.push_back((uint32_t) (v - start.back()));
+        }
+    }
+    if (open) stride.push_back((uint32_t) (n - start.back()));
+    c->roi_start[target]  = std::move(start);
+    c->roi_stride[target] = std::move(stride);
+    c->roi_set[target]    = true;
+    return roi_upload(c);
 }
 
 int

@@ -106,6 +106,11 @@ struct transport_params {
     const float* bl_edges;
     int          bl_n_edges;
     const float* bl_density;
+    // ROIs: NULL is DIRECT for both (the regular scorers score voxels v > 0,
+    // the statistics pair every voxel); otherwise one byte per voxel, bit 0
+    // set where the regular scorers score, bit 1 where the statistics pair
+    // does (the host expands CONTOUR runs, or DIRECT for a target without one)
+    const uint8_t*  roi_flags;
     // per-step trace (verification)
     int64_t* trace_cnb;
     float*   trace_rec;

@@ -515,6 +515,12 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
                   uint64_t dij_item, uint64_t scen_off, pending& A) {
     const int     n   = P.n_scorers;
     const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT
+    // regular scorers score through roi_->idx(cnb) > 0, the statistics pair
+    // through get_mask_idx(cnb) >= 0 (mqi_transport.hpp:342-364); a CONTOUR
+    // ROI's answer per voxel is in roi_flags
+    const uint32_t fl      = P.roi_flags ? (uint32_t) P.roi_flags[cnb] : (roi > 0 ? 3u : 2u);
+    const bool     in_reg  = fl & 1u;
+    const bool     in_stat = fl & 2u;
     // Both loops run over the fixed maximum with constant indices so that the
     // scorer table stays in kernel arguments / registers (a runtime index
     // into it would move the whole argument block to private memory).
@@ -553,7 +559,7 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             if (s >= n) continue;
             const int  k    = P.kinds[s];
             const bool stat = (k == MQI_KIND_STAT_DOSE || k == MQI_KIND_STAT_DOSE2);
-            if (!(stat ? roi >= 0 : roi > 0) || v[s] <= 0) continue;
+            if (!(stat ? in_stat : in_reg) || v[s] <= 0) continue;
             if (k == MQI_KIND_DIJ)
                 dij_insert(P, dij_item, (uint32_t) cnb, v[s]);
             else if (s < MQI_DEFER)
@@ -570,16 +576,16 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
         const bool stat = (k == MQI_KIND_STAT_DOSE || k == MQI_KIND_STAT_DOSE2);
         int        reps = 0;
         if (P.loops == MQI_LOOPS_ONCE) {
-            reps = (stat ? roi >= 0 : roi > 0) ? 1 : 0;
+            reps = (stat ? in_stat : in_reg) ? 1 : 0;
         } else if (P.loops == MQI_LOOPS_PLAIN) {
             // mqi_transport.hpp:204-225: scorers 0..n-3 in both loops
-            reps = (roi > 0) ? ((s < n - 2) ? 2 : 1) : 0;
+            reps = in_reg ? ((s < n - 2) ? 2 : 1) : 0;
         } else {
             // mqi_transport.hpp:343-364: last two through get_mask_idx >= 0
             if (s < n - 2)
-                reps = roi > 0 ? 1 : 0;
+                reps = in_reg ? 1 : 0;
             else if (n >= 2)
-                reps = roi >= 0 ? 1 : 0;
+                reps = in_stat ? 1 : 0;
         }
         if (v[s] <= 0) continue;
         if (k == MQI_KIND_DIJ) {

@@ -1082,6 +1082,42 @@ struct Scorers {
     }
 };
 
+// The scorers' ROI ([0]) and the statistics ROI ([1]) set by oracle_set_roi:
+// empty is DIRECT (roi_t(mqi::DIRECT, n), idx(v) = v), otherwise the CONTOUR
+// runs of mask_reader::mask_to_roi (mqi_file_handler.hpp:177-217).
+struct RoiRuns {
+    std::vector<uint32_t> start, stride;
+};
+RoiRuns g_roi[2];
+
+// roi_t::lower_bound_cpp (mqi_roi.hpp:147-166): the first run whose start
+// exceeds v
+inline int32_t
+roi_lower_bound(const RoiRuns& r, uint32_t v) {
+    int32_t first = 0, count = (int32_t) r.start.size();
+    while (count > 0) {
+        int32_t step = count / 2, it = first + step;
+        if (r.start[it] <= v) {
+            first = it + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    return first;
+}
+
+// idx(v) > 0 (scorers) and get_mask_idx(v) >= 0 (statistics pair) of a
+// CONTOUR ROI (idx_contour, get_contour_idx, mqi_roi.hpp:71-84,129-138):
+// the run before lower_bound holds v when v - start < stride.  For v before
+// the first run get_contour_idx answers -1; idx_contour reads start_[-1]
+// there, which is taken as outside as well.
+inline bool
+roi_contour_in(const RoiRuns& r, uint32_t v) {
+    const int32_t c = roi_lower_bound(r, v) - 1;
+    return c >= 0 && v - r.start[c] < r.stride[c];
+}
+
 struct StepLog {
     int64_t  cap = 0, n = 0;
     int64_t* cnb = nullptr;
@@ -1151,28 +1187,32 @@ struct Transport {
         }
         const int n  = (int) sc->kinds.size();
         const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT: idx(v) = v
+        // regular scorers go through roi_->idx(cnb) > 0, the statistics pair
+        // through get_mask_idx(cnb) >= 0
+        const bool in_reg  = g_roi[0].start.empty() ? roi > 0 : roi_contour_in(g_roi[0], (uint32_t) cnb);
+        const bool in_stat = g_roi[1].start.empty() ? roi >= 0 : roi_contour_in(g_roi[1], (uint32_t) cnb);
         if (kernel == 2) {
             // each scorer once: regular scorers through idx(cnb) > 0, the two
             // stat scorers through get_mask_idx(cnb) >= 0 (what the _stat
             // kernel does; for fewer than three scorers also the plain kernel)
             for (int s = 0; s < n; ++s) {
                 const bool stat = sc->kinds[s] == 3 || sc->kinds[s] == 4;
-                if (stat ? roi >= 0 : roi > 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
+                if (stat ? in_stat : in_reg) insert(s, cnb, hit(sc->kinds[s], t, cnb));
             }
             return;
         }
         // first loop: scorers 0 .. n-3 (mqi_transport.hpp:204-214 / 343-352)
         for (int s = 0; s < n - 2; ++s)
-            if (roi > 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
+            if (in_reg) insert(s, cnb, hit(sc->kinds[s], t, cnb));
         if (kernel == 0) {
             // plain kernel: every scorer again (mqi_transport.hpp:215-225)
             for (int s = 0; s < n; ++s)
-                if (roi > 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
+                if (in_reg) insert(s, cnb, hit(sc->kinds[s], t, cnb));
         } else if (n >= 2) {
             // _stat kernel: the last two scorers through get_mask_idx >= 0 (:353-364);
             // with fewer than two scorers the uint8_t start index wraps and the loop is empty
             for (int s = n - 2; s < n; ++s)
-                if (roi >= 0) insert(s, cnb, hit(sc->kinds[s], t, cnb));
+                if (in_stat) insert(s, cnb, hit(sc->kinds[s], t, cnb));
         }
     }
 
@@ -1520,6 +1560,34 @@ oracle_set_beamline(int n, const int* dims, const float* edges, const float* den
         g_beamline.dens.emplace_back(dens + dofs, dens + dofs + nv);
         dofs += nv;
         g_beamline.xf.insert(g_beamline.xf.end(), xf + 12 * k, xf + 12 * k + 12);
+    }
+}
+
+// ROI of the scorers (target 0) or the statistics pair (target 1) for the
+// next transports: mask = NULL is DIRECT; otherwise mask_to_roi's runs
+// (mqi_file_handler.hpp:185-203): in voxel order a 1 opens a run, a 0 closes
+// it, other values change nothing.  A run open at the last voxel is closed
+// there (the reference leaves that stride unset).
+void
+oracle_set_roi(int target, int64_t nvox, const uint8_t* mask) {
+    RoiRuns& r = g_roi[target];
+    r          = RoiRuns{};
+    if (!mask) return;
+    bool started = false;
+    for (int64_t v = 0; v < nvox; ++v) {
+        if (mask[v] == 1 && !started) {
+            started = true;
+            r.start.push_back((uint32_t) v);
+        }
+        if (mask[v] == 0 && started) {
+            started = false;
+            r.stride.push_back((uint32_t) v - r.start.back());
+        }
+    }
+    if (started) r.stride.push_back((uint32_t) nvox - r.start.back());
+    if (r.start.empty()) {   // nothing scores: one empty run keeps the ROI CONTOUR
+        r.start.push_back(0);
+        r.stride.push_back(0);
     }
 }
 

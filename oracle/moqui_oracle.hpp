@@ -25,6 +25,8 @@ extern "C" {
 
 void oracle_set_beamline(int n, const int* dims, const float* edges, const float* dens, const float* xf);
 
+void oracle_set_roi(int target, int64_t nvox, const uint8_t* mask);
+
 int oracle_transport(int prec, int rng, int math, int debug, uint64_t master_seed, int nx, int ny,
                      int nz, const float* xe, const float* ye, const float* ze,
                      const float* density, const float* rot9, const float* t3, int64_t n_hist,

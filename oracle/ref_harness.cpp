@@ -117,6 +117,17 @@ struct beamline_child {
 };
 std::vector<beamline_child*> g_beamline;
 
+// ROIs set by ref_set_roi: [0] for the regular scorers, [1] for the
+// statistics pair; unset is roi_t(mqi::DIRECT, nvox).  The runs are what
+// mask_reader::mask_to_roi builds (mqi_file_handler.hpp:177-217; that file
+// pulls in GDCM, so the loop is restated in ref_set_roi), the lookups are the
+// reference's own roi_t.
+struct roi_runs {
+    bool                  set = false;
+    std::vector<uint32_t> start, stride, acc;
+};
+roi_runs g_roi[2];
+
 struct world_t {
     mqi::node_t<R>                     world;
     mqi::node_t<R>                     child;
@@ -126,6 +137,7 @@ struct world_t {
     std::vector<mqi::key_value*>       tables;
     std::vector<mqi::roi_t*>           rois;
     std::vector<uint32_t>              ones;
+    mqi::roi_t*                        stat_roi = nullptr;
 };
 
 mqi::grid3d<mqi::density_t, R>*
@@ -165,26 +177,34 @@ build_world(world_t& w, mqi::grid3d<mqi::density_t, R>* geo, int mask) {
     w.geo               = geo;
     auto* direct        = new mqi::roi_t(mqi::DIRECT, nvox);
     w.rois.push_back(direct);
+    mqi::roi_t* roi[2]  = { direct, direct };
+    for (int k = 0; k < 2; ++k)
+        if (g_roi[k].set) {
+            roi[k] = new mqi::roi_t(mqi::CONTOUR, nvox, (int32_t) g_roi[k].start.size(), g_roi[k].start.data(),
+                                    g_roi[k].stride.data(), g_roi[k].acc.data());
+            w.rois.push_back(roi[k]);
+        }
+    w.stat_roi = roi[1];
     if (mask & SC_LOG) {
         w.ones.assign(nvox, 1u);
         auto* ind = new mqi::roi_t(mqi::INDIRECT, nvox, nvox, w.ones.data());
         w.rois.push_back(ind);
         add_scorer(w, "log", nvox, log_hit, ind);
     } else {
-        if (mask & SC_DOSE) add_scorer(w, "Dose", nvox, mqi::dose_to_water<R>, direct);
+        if (mask & SC_DOSE) add_scorer(w, "Dose", nvox, mqi::dose_to_water<R>, roi[0]);
         if (mask & SC_LETD) {
-            add_scorer(w, "LETd_numer", nvox, mqi::LETd_weight1<R>, direct);
-            add_scorer(w, "LETd_denom", nvox, mqi::LETd_weight2<R>, direct);
+            add_scorer(w, "LETd_numer", nvox, mqi::LETd_weight1<R>, roi[0]);
+            add_scorer(w, "LETd_denom", nvox, mqi::LETd_weight2<R>, roi[0]);
         }
         if (mask & SC_LETT) {
-            add_scorer(w, "LETt_numer", nvox, mqi::LETt_weight1<R>, direct);
-            add_scorer(w, "LETt_denom", nvox, mqi::LETt_weight2<R>, direct);
+            add_scorer(w, "LETt_numer", nvox, mqi::LETt_weight1<R>, roi[0]);
+            add_scorer(w, "LETt_denom", nvox, mqi::LETt_weight2<R>, roi[0]);
         }
-        if (mask & SC_EDEP) add_scorer(w, "Edep", nvox, mqi::energy_deposit<R>, direct);
-        if (mask & SC_DM) add_scorer(w, "Dm", nvox, mqi::dose_to_medium<R>, direct);
+        if (mask & SC_EDEP) add_scorer(w, "Edep", nvox, mqi::energy_deposit<R>, roi[0]);
+        if (mask & SC_DM) add_scorer(w, "Dm", nvox, mqi::dose_to_medium<R>, roi[0]);
         if (mask & SC_STAT) {
-            add_scorer(w, "Dose_stat", nvox, mqi::dose_to_water<R>, direct);
-            add_scorer(w, "DoseSquare_stat", nvox, mqi::dose_to_water_square<R>, direct);
+            add_scorer(w, "Dose_stat", nvox, mqi::dose_to_water<R>, roi[1]);
+            add_scorer(w, "DoseSquare_stat", nvox, mqi::dose_to_water_square<R>, roi[1]);
         }
     }
     w.child.geo        = geo;
@@ -208,6 +228,36 @@ free_world(world_t& w) {
     }
     for (auto* t : w.tables) delete[] t;
     for (auto* r : w.rois) delete r;
+}
+
+// ROI of the scorers (target 0) or the statistics pair (target 1) of the
+// next transports; mask = NULL restores DIRECT.  mask_to_roi's loop: in voxel
+// order a 1 opens a run, a 0 closes it and appends its stride and the
+// accumulated stride.  Returns -1 for a mask whose last run is still open at
+// the last voxel (the reference leaves that stride unset) or that has no run.
+extern "C" int
+ref_set_roi(int target, int64_t nvox, const uint8_t* mask) {
+    roi_runs& r = g_roi[target];
+    r           = roi_runs{};
+    if (!mask) return 0;
+    bool started = false;
+    for (int64_t v = 0; v < nvox; ++v) {
+        if (mask[v] == 1 && !started) {
+            started = true;
+            r.start.push_back((uint32_t) v);
+        }
+        if (mask[v] == 0 && started) {
+            started = false;
+            r.stride.push_back((uint32_t) v - r.start.back());
+            r.acc.push_back((r.acc.empty() ? 0u : r.acc.back()) + r.stride.back());
+        }
+    }
+    if (started || r.start.empty()) {
+        r = roi_runs{};
+        return -1;
+    }
+    r.set = true;
+    return 0;
 }
 
 extern "C" void
@@ -480,10 +530,13 @@ ref_transport(int nx, int ny, int nz, const float* xe, const float* ye, const fl
             if (!out[k]) continue;
             double* o = out[k];
             for (uint32_t i = 0; i < nvox; ++i) o[i] = 0.0;
+            // the statistics pair of a CONTOUR ROI is keyed by the compact
+            // index get_mask_idx gave; get_org_idx maps it back to the voxel
+            const bool compact = (mask & SC_STAT) && k >= n_sc - 2 && g_roi[1].set;
             for (auto& w : worlds) {
                 const mqi::key_value* kv = w.tables[k];
                 for (uint32_t i = 0; i < nvox; ++i)
-                    if (kv[i].key1 != mqi::empty_pair) o[i] += kv[i].value;
+                    if (kv[i].key1 != mqi::empty_pair) o[compact ? (uint32_t) w.stat_roi->get_org_idx(i) : i] += kv[i].value;
             }
         }
     }

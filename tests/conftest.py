@@ -22,7 +22,10 @@ def _session_ctx():
 
 @pytest.fixture
 def ctx(_session_ctx):
-    """The session's context with the nominal scenario only and no beamline."""
+    """The session's context with the nominal scenario only, no beamline and
+    DIRECT scoring."""
     _session_ctx.set_scenarios()
     _session_ctx.set_beamline()
+    _session_ctx.set_roi(0)
+    _session_ctx.set_roi(1)
     return _session_ctx

@@ -51,3 +51,40 @@ def aperture(radius=8.0, half=15.0, thick=10.0, centre=(0.0, 0.0, 10.0), rot_deg
     c, s = np.float32(np.cos(a)), np.float32(np.sin(a))
     rot = np.array([c, -s, 0, s, c, 0, 0, 0, 1], np.float32)
     return rb.Grid(xe, xe.copy(), ze, dens, rot9=rot, t3=np.array(centre, np.float32))
+
+
+def roi_masks(nx, ny, nz):
+    """Two ROI masks (x fastest) in the form mask_reader::mask_total takes,
+    the sum of structure masks: for the scorers a cylinder of radius 5 voxels
+    with an overlapping inner cylinder (value 2) and a lone run of 2s on the
+    axis, upstream of the cylinder, that no 1 opens; for the statistics pair a cylinder of radius 3 with a column of
+    2s right after the last 1 of the centre row, which continue its run.  Voxel 0 is 1 in the first, so the reference's
+    idx_contour never looks before the first run, and the last voxel is 0 in
+    both, so every run closes."""
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r2 = (i - nx // 2) ** 2 + (j - ny // 2) ** 2
+    m0 = np.zeros((nz, ny, nx), np.uint8)
+    m0[(r2 < 25) & (k >= 20) & (k < 160)] = 1
+    m0[(r2 < 4) & (k >= 60) & (k < 100)] = 2
+    m0[nz - 10, ny // 2, nx // 2 - 2:nx // 2 + 3] = 2
+    m0.flat[0] = 1
+    m1 = np.zeros((nz, ny, nx), np.uint8)
+    m1[(r2 < 9) & (k >= 30) & (k < 140)] = 1
+    m1[(i == nx // 2 + 3) & (j == ny // 2) & (k >= 30) & (k < 140)] = 2
+    assert m0.flat[-1] == 0 and m1.flat[-1] == 0
+    return m0.ravel(), m1.ravel()
+
+
+def roi_inside(mask):
+    """Voxels a CONTOUR ROI built from mask covers, by walking the mask: a 1
+    opens a run, a 0 closes it (the 0 itself outside), other values keep the
+    state; a run still open at the end covers the rest."""
+    inside = np.zeros(mask.size, bool)
+    state = False
+    for v, m in enumerate(mask.tolist()):
+        if m == 1:
+            state = True
+        elif m == 0:
+            state = False
+        inside[v] = state
+    return inside

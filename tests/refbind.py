@@ -255,6 +255,25 @@ def _beamline_args(children):
             cat(dens, np.float32), cat(xf, np.float32))
 
 
+def set_roi(target, nvox, mask=None):
+    """ROI (target 0: scorers, 1: statistics pair) of the oracle's and, where
+    it is built, the reference harness's next transports; None: DIRECT.  The
+    harness refuses a mask whose last run is open at the last voxel or that
+    has no run (the reference leaves those undefined)."""
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8).reshape(-1)
+    mp = ptr(m, ctypes.POINTER(ctypes.c_uint8))
+    o = oracle()
+    o.oracle_set_roi.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]
+    o.oracle_set_roi(target, nvox, mp)
+    if have_reference():
+        for debug in (0, 1):
+            lib = reference(debug)
+            lib.ref_set_roi.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8)]
+            lib.ref_set_roi.restype = ctypes.c_int
+            if lib.ref_set_roi(target, nvox, mp) != 0:
+                raise ValueError("the reference harness leaves this mask undefined")
+
+
 def set_beamline(children):
     """Beamline children (before the phantom) of the oracle's and, where it
     is built, the reference harness's next transports; [] clears them."""

@@ -272,3 +272,72 @@ def test_exact_lett_edep_dm_match_oracle(ctx, debug):
     with pytest.raises(Exception):
         ctx.set_scorers(rb.SC_DOSE | rb.SC_LETD | rb.SC_LETT | rb.SC_STAT)
     ctx.set_scorers(rb.SC_DOSE)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("debug", [0, 1])
+def test_exact_roi_masks_match_oracle(ctx, debug):
+    """CONTOUR ROIs (mqi_set_roi) for the scorers and the statistics pair:
+    the verification kernels equal the oracle (whose gates equal the
+    reference's roi_t, tests/test_oracle_vs_reference.py) up to FP64
+    summation order; the production kernels score the same voxels, every
+    array is zero off its ROI and the Dij rows hold only ROI voxels."""
+    g = phantoms.slab_box(24, 24, 200)
+    nvox = g.nx * g.ny * g.nz
+    sp = phantoms.spots([150.0, 175.0], 300, sigma=4.0, sigma_e_frac=0.01, positions=[(0, 0, 0.5), (3, -2, 0.5)])
+    m0, m1 = phantoms.roi_masks(g.nx, g.ny, g.nz)
+    in0, in1 = phantoms.roi_inside(m0), phantoms.roi_inside(m1)
+    mask = rb.SC_DOSE | rb.SC_LETD | rb.SC_STAT
+    rb.set_roi(0, nvox, m0)
+    rb.set_roi(1, nvox, m1)
+    try:
+        ref, ntr = rb.oracle_run_spots(g, sp, mask=mask, kernel=2, prec=32, rng=1, math=1, debug=debug,
+                                       master_seed=31, threads=4)
+    finally:
+        rb.set_roi(0, nvox)
+        rb.set_roi(1, nvox)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(mask)
+    with pytest.raises(ValueError):
+        ctx.set_roi(0, m0[:-1])
+    with pytest.raises(Exception):
+        ctx.set_roi(2, m0)
+    ctx.set_roi(0, m0)
+    ctx.set_roi(1, m1)
+    res = ctx.run(ctx.config(seed=31, exact=True, batched=True, physics_debug=bool(debug)))
+    assert res.histories == ntr == 600
+    for k in range(5):
+        got, want = ctx.scorer(k), ref[k]
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got), fin)
+        assert np.allclose(got[fin], want[fin], rtol=1e-9, atol=0), k
+        inside = in1 if k >= 3 else in0
+        assert not np.any(got[~inside]) and np.any(got[inside] != 0), k
+    if debug:
+        return
+    # production kernels: gated equals ungated on the ROI (same histories,
+    # scoring does not steer transport), zero elsewhere
+    ctx.zero_scorers()
+    ctx.run(ctx.config(seed=5, batched=True))
+    gated = [ctx.scorer(k) for k in range(5)]
+    ctx.set_roi(0)
+    ctx.set_roi(1)
+    ctx.zero_scorers()
+    ctx.run(ctx.config(seed=5, batched=True))
+    for k in range(5):
+        inside = (in1 if k >= 3 else in0).copy()
+        if k < 3:
+            inside[0] = False
+        free = ctx.scorer(k)
+        assert np.allclose(gated[k][inside], free[inside], rtol=1e-9, atol=0), k
+        assert not np.any(gated[k][~(in1 if k >= 3 else in0)]), k
+    # Dij goes through the scorers' ROI
+    ctx.set_roi(0, m0)
+    ctx.set_scorers(rb.SC_DOSE)
+    ctx.zero_scorers()
+    ctx.run(ctx.config(seed=6, batched=True, dij=True, dij_capacity=1 << 15))
+    ctx.dij_build(0.0)
+    indptr, idx, _, _ = ctx.dij_csr()
+    assert indptr[-1] > 0 and np.all(in0[idx[:indptr[-1]]])
+    ctx.set_roi(0)

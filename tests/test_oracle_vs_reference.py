@@ -221,3 +221,42 @@ def test_grid_probe_bit_exact():
         assert np.array_equal(a1.view(np.uint32), b1.view(np.uint32)), op
         assert np.array_equal(a2.view(np.uint32), b2.view(np.uint32)), op
         assert np.array_equal(a3, b3), op
+
+
+@needs_ref
+@pytest.mark.parametrize("debug", [1, 0])
+def test_roi_masks_bit_exact(debug):
+    """CONTOUR ROIs from mask_to_roi's runs for the scorers and the statistics
+    pair: the reference's roi_t gates (idx > 0 for the scorers, get_mask_idx
+    >= 0 for the statistics pair, whose compact index the harness maps back
+    with get_org_idx) and the oracle's agree bit for bit, and the ROI covers
+    exactly the voxels a walk over the mask finds."""
+    g = phantoms.slab_box(24, 24, 200)
+    nvox = g.nx * g.ny * g.nz
+    vtx, seeds = _spot_vertices(debug, 160.0, 150, sigma_e=1.6)
+    m0, m1 = phantoms.roi_masks(g.nx, g.ny, g.nz)
+    in0, in1 = phantoms.roi_inside(m0), phantoms.roi_inside(m1)
+    cases = [(rb.SC_DOSE | rb.SC_LETD, 0), (rb.SC_DOSE | rb.SC_LETT | rb.SC_STAT, 1)]
+    free = {c: rb.ref_transport(debug, g, vtx, seeds, mask=c[0], kernel=c[1])[0] for c in cases}
+    lone = (m0 == 2) & ~in0   # 2s that no 1 opens: outside, yet in the beam
+    assert lone.sum() == 5 and np.all(free[cases[0]][0][lone] > 0)
+    rb.set_roi(0, nvox, m0)
+    rb.set_roi(1, nvox, m1)
+    try:
+        for mask, kernel in cases:
+            ra, nr = rb.ref_transport(debug, g, vtx, seeds, mask=mask, kernel=kernel)
+            oa, no = rb.oracle_transport(g, vtx, seeds, mask=mask, kernel=kernel, debug=debug)
+            assert nr == no == 150
+            for k, (a, b) in enumerate(zip(ra, oa)):
+                assert np.array_equal(_bits(a), _bits(b)), k
+                stat = mask & rb.SC_STAT and k >= len(ra) - 2
+                inside = (in1 if stat else in0).copy()
+                if not stat:
+                    inside[0] = False   # DIRECT leaves voxel 0 out of the scorers
+                f = free[(mask, kernel)][k]
+                assert np.array_equal(_bits(a[inside]), _bits(f[inside])), k
+                assert not np.any(a[~(in1 if stat else in0)]), k
+                assert np.any(a[inside] != 0), k
+    finally:
+        rb.set_roi(0, nvox)
+        rb.set_roi(1, nvox)
