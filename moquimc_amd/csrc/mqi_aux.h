@@ -29,7 +29,9 @@ int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tm
 // CSR-Adaptive segment sizes (entries of one long-row piece; entries and rows
 // of one packed segment)
 #define MQI_SEG_LONG 16384
+#ifndef MQI_SEG_SHORT
 #define MQI_SEG_SHORT 1024
+#endif
 #define MQI_SEG_ROWS 1024
 int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
                  const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,

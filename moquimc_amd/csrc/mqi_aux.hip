@@ -296,7 +296,9 @@ segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, i
 //                    in order.
 //   packed segment   whole rows, <= MQI_SEG_SHORT entries and <= MQI_SEG_ROWS
 //                    rows: the wave forms the products into LDS with
-//                    coalesced loads, then each lane sums one row.
+//                    coalesced loads, then each row is summed by L lanes
+//                    (the largest power of two with L * rows <= 64; strided
+//                    parts, then shuffles within the group).
 // Every sum runs in a fixed order, so results do not depend on scheduling.
 // seg_info[s] = first row | (partial << 31); seg_k[s] = first nonzero;
 // entry n_seg of both is the sentinel (rows, nnz).
@@ -357,14 +359,39 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             if (lane == 0) part[s] = acc;
         } else {
             const int64_t r0 = info, r1 = seg_info[s + 1] & 0x7fffffffu;
-            for (int64_t k = kb + lane; k < ke; k += 64) buf[k - kb] = (double) val[k] * (double) x[col[k]];
+            // products into LDS, four coalesced loads in flight per lane
+            int64_t k = kb + lane;
+            for (; k + 192 < ke; k += 256) {
+                float   v[4];
+                int32_t cc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    v[u]  = val[k + 64 * u];
+                    cc[u] = col[k + 64 * u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) buf[k + 64 * u - kb] = (double) v[u] * (double) x[cc[u]];
+            }
+            for (; k < ke; k += 64) buf[k - kb] = (double) val[k] * (double) x[col[k]];
             __syncthreads();
-            for (int64_t r = r0 + lane; r < r1; r += 64) {
+            // L lanes per row (a power of two, L * rows <= 64): each sums a
+            // strided part, then the group adds its parts with shuffles
+            const int64_t nr = r1 - r0;
+            int           L  = 1;
+            while (L < 64 && 2 * L * nr <= 64) L *= 2;
+            const int sub = lane & (L - 1);
+            for (int64_t rb = r0; rb < r1; rb += 64 / L) {
+                const int64_t r   = rb + lane / L;
                 double        acc = 0.0;
-                const int64_t e   = indptr[r + 1] - kb;
-                for (int64_t k = indptr[r] - kb; k < e; ++k) acc += buf[k];
-                const double out = alpha * acc;
-                y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+                if (r < r1) {
+                    const int64_t e = indptr[r + 1] - kb;
+                    for (int64_t k = indptr[r] - kb + sub; k < e; k += L) acc += buf[k];
+                }
+                for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+                if (r < r1 && sub == 0) {
+                    const double out = alpha * acc;
+                    y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+                }
             }
             __syncthreads();
         }
