@@ -1,6 +1,7 @@
 """Benchmark: proton transport histories/s on one MI355X, batched persistent
 launch vs plain per-spot launches, on the 2000-spot synthetic CT plan; plus
-the Dij adjoint (D^T g) bandwidth on the 5000-spot Dij workload.
+the Dij adjoint (D^T g) bandwidth on the 5000-spot Dij workload, the
+9-scenario robust Dij leg, and the water and slab phantom workloads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -156,6 +157,39 @@ def adjoint_bench(ctx_cls, reps=20):
                                  "kernel": "k_csr_adaptive + k_csr_split_rows (D^T g)", "algorithmic_bytes": int(alg_bytes)}}
 
 
+def phantom_bench(ctx_cls, reps=3):
+    """The two phantom workloads: water (200x200x350, one 200 MeV spot, 1e5
+    histories, dose, phantom-build physics) and slabs (25 spots 70-230 MeV,
+    1e6 histories each, dose + LETd).  One batched launch per run, launch
+    shape from the autotuner; histories/s over the wall time of the best of
+    reps runs (each run synchronised)."""
+    from moquimc_amd import workloads
+    out = {}
+    for key, w, sample in (("water", workloads.water(), 1.0), ("slabs", workloads.slabs(), 0.1)):
+        ctx = ctx_cls(0)
+        w.load(ctx)
+        cfg = ctx.config(seed=5, batched=True, physics_debug=w.debug)
+        cfg, _ = ctx.autotune(cfg, sample=sample)
+        ctx.zero_scorers()
+        ctx.run(cfg)
+        best, kms = None, None
+        for i in range(reps):
+            ctx.zero_scorers()
+            cfg.hist_offset = (i + 1) * (1 << 24)
+            t0 = time.perf_counter()
+            r = ctx.run(cfg)
+            dt = time.perf_counter() - t0
+            if best is None or dt < best:
+                best, kms = dt, r.kernel_ms
+        ctx.close()
+        out[key] = {"workload": w.name, "physics": "phantom build" if w.debug else "release",
+                    "histories": int(w.histories), "wall_s": round(best, 4),
+                    "histories_per_s": round(w.histories / best, 1),
+                    "kernel_histories_per_s": round(w.histories / (kms / 1e3), 1)}
+        log("%s: %.4f s wall, %.0f histories/s" % (key, best, w.histories / best))
+    return {"phantoms": out}
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its
     dispatches against nbytes give the scale of the HBM counters."""
@@ -246,6 +280,8 @@ def main():
     ap.add_argument("--exact", action="store_true", help="time the verification kernels instead")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters instead of the autotuner's")
     ap.add_argument("--adjoint-only", action="store_true", help="only the Dij workload and its products")
+    ap.add_argument("--no-phantoms", action="store_true", help="skip the water and slab phantom leg")
+    ap.add_argument("--phantoms-only", action="store_true", help="only the water and slab phantom leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the reference kernel of the HBM counter runs)")
     args = ap.parse_args()
@@ -264,6 +300,10 @@ def main():
     if args.adjoint_only:
         from moquimc_amd import _abi
         print(json.dumps(adjoint_bench(_abi.Context)), flush=True)
+        return
+    if args.phantoms_only:
+        from moquimc_amd import _abi
+        print(json.dumps(phantom_bench(_abi.Context)), flush=True)
         return
 
     from moquimc_amd import _abi, workloads
@@ -343,6 +383,11 @@ def main():
             out.update(robust_bench(_abi.Context))
         except Exception as e:  # reported, not hidden
             out["robust_error"] = repr(e)
+    if not args.no_phantoms:
+        try:
+            out.update(phantom_bench(_abi.Context))
+        except Exception as e:  # reported, not hidden
+            out["phantoms_error"] = repr(e)
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)
         try:
