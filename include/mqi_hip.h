@@ -215,6 +215,17 @@ int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, vo
  * Set after mqi_set_grid: setting a grid clears both masks. */
 int mqi_set_roi(mqi_ctx* ctx, int target, const uint8_t* mask);
 
+/* The same from structure mask files: tps_env's ScoringMask "Mask" list
+ * (target 0) or StatROIMaskFilename list (target 1), mqi_tps_env.hpp:
+ * 224-232,290-291,776-805.  Each file is a MetaImage with one byte per voxel
+ * after the header (ElementDataFile = LOCAL), read as
+ * mask_reader::read_mha_file does (mqi_file_handler.hpp:39-89); the masks
+ * are summed into mask_total (read_mask_files, :92-113), which then goes
+ * through mqi_set_roi.  A file whose DimSize differs from the grid's, whose
+ * ElementType is not MET_UCHAR / MET_CHAR or which holds a value other than
+ * 0 or 1 is refused with MQI_E_IO.  n = 0 restores DIRECT. */
+int mqi_set_roi_mha(mqi_ctx* ctx, int target, int n, const char* const* paths);
+
 /* Beamline children transported before the patient grid, in order (the
  * world of tps_env::setup_world, mqi_tps_env.hpp:713-753: range shifter and
  * voxelised aperture nodes, then the phantom; transport loop

@@ -90,6 +90,7 @@ def lib():
             "mqi_set_grid_ct_mha": (i, [vp, ctypes.c_char_p, f, vp]),
             "mqi_set_beamline": (i, [vp, i, vp]),
             "mqi_set_roi": (i, [vp, i, vp]),
+            "mqi_set_roi_mha": (i, [vp, i, i, ctypes.POINTER(ctypes.c_char_p)]),
             "mqi_save_scorer": (i, [vp, i, i, ctypes.c_char_p, ctypes.c_char_p, i, d]),
         }
         for name, (res, args) in sig.items():
@@ -105,7 +106,7 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
-            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi"]
+            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha"]
 
 
 class GridDesc(ctypes.Structure):
@@ -214,6 +215,12 @@ class Context:
             if m.size != self.nvox:
                 raise ValueError("mask has %d voxels, the grid %d" % (m.size, self.nvox))
         self._chk(self._L.mqi_set_roi(self.h, int(target), _p(m)), "mqi_set_roi")
+
+    def set_roi_mha(self, target, paths=()):
+        """ROI from structure mask files (MetaImage, uint8 after the header),
+        summed as the reference's mask reader does; () restores DIRECT."""
+        arr = (ctypes.c_char_p * max(1, len(paths)))(*[os.fsencode(p) for p in paths])
+        self._chk(self._L.mqi_set_roi_mha(self.h, int(target), len(paths), arr), "mqi_set_roi_mha")
 
     def set_grid_ct_mha(self, path, density_scale=1.0, shift=None):
         """CT from a MetaImage file (float32 HU after the header)."""

@@ -1474,6 +1474,28 @@ mqi_set_grid_ct_mha(mqi_ctx* c, const char* path, float density_scale, const flo
 }
 
 int
+mqi_set_roi_mha(mqi_ctx* c, int target, int n, const char* const* paths) {
+    if (!c) return MQI_E_ARG;
+    if (target != 0 && target != 1) return fail(c, MQI_E_ARG, "roi target %d (0 scorers, 1 statistics)", target);
+    if (n == 0) return mqi_set_roi(c, target, nullptr);
+    if (n < 0 || !paths) return fail(c, MQI_E_ARG, "mask files: %d", n);
+    int r = ready(c);
+    if (r) return r;
+    std::vector<std::string> p;
+    for (int k = 0; k < n; ++k) {
+        if (!paths[k]) return fail(c, MQI_E_ARG, "mask file %d is NULL", k);
+        p.emplace_back(paths[k]);
+    }
+    std::vector<uint8_t> total;
+    try {
+        total = mqi_hip::io::read_mask_total(p, c->nx, c->ny, c->nz);
+    } catch (const std::exception& e) {
+        return fail(c, MQI_E_IO, "%s", e.what());
+    }
+    return mqi_set_roi(c, target, total.data());
+}
+
+int
 mqi_dij_save_npz(mqi_ctx* c, int s, const char* path, double scale) {
     if (!c || !path) return MQI_E_ARG;
     if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);

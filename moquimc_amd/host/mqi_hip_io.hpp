@@ -375,6 +375,64 @@ read_mha_ct(const std::string& path, const float* shift = nullptr) {
     return ct;
 }
 
+// A structure mask in MetaImage form, as mask_reader::read_mha_file reads
+// it (mqi_file_handler.hpp:39-89): DimSize, the data after the header
+// (ElementDataFile = LOCAL), one byte per voxel, x fastest.  Different on
+// purpose: the mask must have the grid's dimensions (the reference reads
+// the mask's own DimSize worth of bytes and then indexes it over the CT),
+// an ElementType other than MET_UCHAR / MET_CHAR is refused, and so is a
+// value other than 0 or 1 (the reference asserts 0 <= v <= 1).
+inline std::vector<uint8_t>
+read_mask_mha(const std::string& path, int nx, int ny, int nz) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw io_error("cannot open " + path);
+    int         mx = 0, my = 0, mz = 0;
+    bool        have_dim = false, local = false;
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string  key = trim(line.substr(0, eq));
+        std::istringstream v(trim(line.substr(eq + 1)));
+        if (iequal(key, "DimSize")) {
+            have_dim = (bool) (v >> mx >> my >> mz);
+        } else if (iequal(key, "ElementType")) {
+            std::string t;
+            v >> t;
+            if (t != "MET_UCHAR" && t != "MET_CHAR")
+                throw io_error(path + ": ElementType " + t + " (a mask is MET_UCHAR)");
+        } else if (iequal(key, "ElementDataFile")) {
+            std::string t;
+            v >> t;
+            if (!iequal(t, "LOCAL")) throw io_error(path + ": the data must follow the header (ElementDataFile = LOCAL)");
+            local = true;
+            break;
+        }
+    }
+    if (!have_dim || !local) throw io_error(path + ": DimSize or data missing");
+    if (mx != nx || my != ny || mz != nz) throw io_error(path + ": DimSize differs from the grid's");
+    const size_t         n = (size_t) nx * ny * nz;
+    std::vector<uint8_t> m(n);
+    f.read(reinterpret_cast<char*>(m.data()), (std::streamsize) n);
+    if ((size_t) f.gcount() != n) throw io_error(path + ": fewer voxels than DimSize");
+    for (size_t i = 0; i < n; ++i)
+        if (m[i] > 1) throw io_error(path + ": mask values must be 0 or 1");
+    return m;
+}
+
+// mask_reader::read_mask_files (mqi_file_handler.hpp:92-113): the sum of
+// the structure masks (mask_total), the input of mask_to_roi
+inline std::vector<uint8_t>
+read_mask_total(const std::vector<std::string>& paths, int nx, int ny, int nz) {
+    if (paths.empty()) throw io_error("no mask files");
+    std::vector<uint8_t> total((size_t) nx * ny * nz, 0);
+    for (const std::string& p : paths) {
+        const std::vector<uint8_t> m = read_mask_mha(p, nx, ny, nz);
+        for (size_t i = 0; i < total.size(); ++i) total[i] = (uint8_t) (total[i] + m[i]);
+    }
+    return total;
+}
+
 }   // namespace io
 }   // namespace mqi_hip
 

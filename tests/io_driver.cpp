@@ -4,6 +4,7 @@
 //   io_driver ct <ct.mha> <hu.out> [sx sy sz]  reads a MetaImage CT, prints dims and edges
 //   io_driver dense <dir> <name> <nx> <ny> <nz> <scale> <values.f64> <xe.f32> <ye.f32> <ze.f32>
 //       writes <name>_raw.raw, <name>_mhd.mhd + .raw and <name>_mha.mha
+//   io_driver mask <out.u8> <nx> <ny> <nz> <mask.mha>...   sum of the masks (mask_total) to <out>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
@@ -54,6 +55,14 @@ main(int argc, char** argv) {
             }
             std::ofstream o(argv[3], std::ios::binary);
             o.write(reinterpret_cast<const char*>(ct.hu.data()), (std::streamsize) (ct.hu.size() * 2));
+            return 0;
+        }
+        if (mode == "mask" && argc >= 7) {
+            std::vector<std::string> paths(argv + 6, argv + argc);
+            const std::vector<uint8_t> t =
+              mqi_hip::io::read_mask_total(paths, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+            std::ofstream o(argv[2], std::ios::binary);
+            o.write(reinterpret_cast<const char*>(t.data()), (std::streamsize) t.size());
             return 0;
         }
         if (mode == "dense" && argc == 12) {

@@ -194,3 +194,39 @@ def test_mha_ct_reader_refuses_bad_files(driver, tmp_path):
     data = open(p, "rb").read()
     open(p, "wb").write(data[:-4])   # one voxel short
     assert subprocess.run([driver, "ct", str(p), str(tmp_path / "o.bin")], capture_output=True).returncode == 1
+
+
+def write_mask(path, m, element_type="MET_UCHAR", local=True):
+    """A structure mask as MetaImage (uint8 after the header), shape (nz, ny, nx)."""
+    nz, ny, nx = m.shape
+    hdr = ["ObjectType = Image", "NDims = 3", "BinaryData = True", "BinaryDataByteOrderMSB = False",
+           "CompressedData = False", "TransformMatrix = 1 0 0 0 1 0 0 0 1", "Offset = 0 0 0",
+           "CenterOfRotation = 0 0 0", "AnatomicalOrientation = RAI", "ElementSpacing = 1 1 1",
+           "DimSize = %d %d %d" % (nx, ny, nz), "ElementType = " + element_type,
+           "ElementDataFile = " + ("LOCAL" if local else "mask.raw")]
+    with open(path, "wb") as f:
+        f.write(("\n".join(hdr) + "\n").encode())
+        f.write(np.ascontiguousarray(m, np.uint8).tobytes())
+
+
+def test_mask_files_sum_to_mask_total(driver, tmp_path):
+    """mask_reader::read_mask_files: the structure masks summed voxel by
+    voxel (overlaps give 2); refused: another grid size, a value above 1,
+    a wider element type, data outside the file."""
+    rng = np.random.default_rng(4)
+    shape = (5, 6, 7)
+    a = (rng.random(shape) < 0.3).astype(np.uint8)
+    b = (rng.random(shape) < 0.4).astype(np.uint8)
+    pa, pb, out = tmp_path / "a.mha", tmp_path / "b.mha", tmp_path / "total.u8"
+    write_mask(pa, a)
+    write_mask(pb, b)
+    _run(driver, "mask", out, 7, 6, 5, pa, pb)
+    got = np.fromfile(out, np.uint8)
+    assert np.array_equal(got, (a.astype(int) + b).ravel())
+    assert got.max() == 2
+    bad = tmp_path / "bad.mha"
+    for m, kw, dims in ((a, {}, (7, 6, 4)), (a * 2, {}, (7, 6, 5)), (a, {"element_type": "MET_SHORT"}, (7, 6, 5)),
+                        (a, {"local": False}, (7, 6, 5))):
+        write_mask(bad, m, **kw)
+        r = subprocess.run([driver, "mask", str(out)] + [str(d) for d in dims] + [str(bad)], capture_output=True)
+        assert r.returncode == 1, (kw, dims)
