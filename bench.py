@@ -287,7 +287,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take its own slice "
-              "of the (scenario, spot) work list; see DESIGN.md).", file=sys.stderr)
+              "of the (scenario, spot) work list, and the dose grids would be summed once at the end).", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
