@@ -14,6 +14,8 @@ Fixtures (inputs and the reference's outputs, nothing else):
                             (vertices and seeds included), debug and release
   reference_profiles.npz    (--stats) high-statistics dose of the water and
                             slab workloads for the statistical GPU tests
+  reference_ct.npz          (--ct) a 100-spot sub-plan of the CT workload on
+                            its cropped CT: inputs and the reference's dose
 """
 import argparse
 import os
@@ -245,14 +247,58 @@ def profiles_fixture(threads, only=None):
     return out
 
 
+def ct_fixture(threads, per_spot=10000, n_batches=8):
+    """The 100-spot sub-plan on the cropped CT of the 2000-spot plan
+    (phantoms.ct_crop_plan), release physics, n_batches x per_spot histories
+    per spot.  Vertices from the beamlet sampling of the spot records
+    (oracle_sample_spot with the reference's generator, equal to the
+    reference's sampling bit for bit, tests/test_oracle_vs_reference.py),
+    independent per-history transport seeds as in _batches; density from
+    the reference's hu_to_density.  Stored: the inputs (HU crop, edges,
+    spots) and the dose summed over z (x-y map), its integral depth dose
+    along y and the integral dose, with batch standard errors (Gy per
+    history)."""
+    t0 = time.time()
+    xe, ye, ze, hu, spots = phantoms.ct_crop_plan(per_spot)
+    nz, ny, nx = hu.shape
+    rho = np.zeros(hu.size, np.float32)
+    h = np.ascontiguousarray(hu.ravel())
+    rb.reference(0).ref_hu_to_density(rb.ptr(h, rb.ctypes.c_void_p), rb.ctypes.c_int64(h.size), rb.ptr(rho, rb.c_f))
+    g = rb.Grid(xe, ye, ze, rho)
+    runs = []
+    for b in range(n_batches):
+        vtx, seeds = [], []
+        for k, sp in enumerate(spots):
+            v, _ = rb.oracle_sample_spot(sp, 4000 + b, k, per_spot, rng=0)
+            vtx.append(v)
+            seeds.append(np.random.default_rng([4000 + b, k]).integers(1, 2 ** 31 - 1, size=per_spot).astype(np.int32))
+        arrs, tracked = rb.ref_transport(0, g, np.concatenate(vtx), np.concatenate(seeds), mask=rb.SC_DOSE, kernel=0,
+                                         threads=threads)
+        runs.append(arrs[0].reshape(nz, ny, nx) / tracked)
+        print("  batch %d: %d histories, %.0f s" % (b, tracked, time.time() - t0), flush=True)
+    out = {"c3_xe": xe, "c3_ye": ye, "c3_ze": ze, "c3_hu": hu, "c3_spots": spots,
+           "c3_histories": np.array(n_batches * per_spot * len(spots))}
+    out["c3_xy"], out["c3_xy_se"] = _mean_se([r.sum(0) for r in runs])
+    out["c3_idd"], out["c3_idd_se"] = _mean_se([r.sum((0, 2)) for r in runs])
+    out["c3_total"], out["c3_total_se"] = _mean_se([r.sum() for r in runs])
+    print("c3 done in %.0f s" % (time.time() - t0), flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--stats", action="store_true")
     ap.add_argument("--only", choices=["c2"], default=None, help="with --stats: recompute the slab workload only")
+    ap.add_argument("--ct", action="store_true", help="write reference_ct.npz (the cropped-CT sub-plan)")
+    ap.add_argument("--ct-per-spot", type=int, default=10000)
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     args = ap.parse_args()
     if not rb.have_reference():
         raise SystemExit("build the reference harness first: make -C oracle")
+    if args.ct:
+        out = ct_fixture(args.threads, per_spot=args.ct_per_spot)
+        np.savez_compressed(os.path.join(HERE, "reference_ct.npz"), **out)
+        return
     if args.stats:
         out = profiles_fixture(args.threads, only=args.only)
         np.savez_compressed(os.path.join(HERE, "reference_profiles.npz"), **out)

@@ -88,3 +88,23 @@ def roi_inside(mask):
             state = False
         inside[v] = state
     return inside
+
+
+def ct_crop_plan(per_spot):
+    """The 2000-spot plan's CT (512x512x200 @ 0.98x0.98x2 mm, seeded
+    generator) cropped to the voxels whose centres lie within 50 mm of the
+    axis in x and z (102 x 512 x 50 voxels, every depth), and a 100-spot
+    sub-plan of the same kind: 4 layers (100, 126.7, 153.3, 180 MeV) x the
+    25 hexagonal positions closest to the axis, sigma 4 mm, 1 % energy
+    spread, beam along +y from y = -130 mm.  Returns (xe, ye, ze, hu[nz, ny, nx]
+    int16, spots)."""
+    from moquimc_amd import workloads
+    w = workloads.ct()
+    xc = 0.5 * (w.xe[:-1] + w.xe[1:])
+    zc = 0.5 * (w.ze[:-1] + w.ze[1:])
+    ix = np.flatnonzero(np.abs(xc) < 50.0)
+    iz = np.flatnonzero(np.abs(zc) < 50.0)
+    x0, x1, z0, z1 = ix[0], ix[-1] + 1, iz[0], iz[-1] + 1
+    hu = np.ascontiguousarray(w.hu[z0:z1, :, x0:x1])
+    sp = np.asarray(workloads.ct_plan(4, 25, 100.0, 180.0, lambda r: per_spot, 3), w.spots.dtype)
+    return w.xe[x0:x1 + 1].copy(), w.ye.copy(), w.ze[z0:z1 + 1].copy(), hu, sp

@@ -193,3 +193,56 @@ def test_fp32_depth_dose_against_fp64(ctx):
     assert abs(d_r80) <= 0.05
     assert d_idd <= 0.01
     assert abs(idd.sum() / ref["idd64"].sum() - 1) <= 1e-3
+
+
+GOLD_CT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_ct.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD_CT), reason="reference_ct.npz not generated")
+def test_ct_subplan_against_reference(ctx):
+    """The CT workload's kind of plan on its own CT (release physics): a
+    100-spot sub-plan (4 layers 100-180 MeV x 25 spots) on the 2000-spot
+    plan's CT cropped to 102 x 512 x 50 voxels, reference 8 batches x 2e4
+    histories per spot (reference_ct.npz, inputs included).  The batched
+    production kernels run 8 batches x 1e5 per spot; compared: the dose
+    summed over z (the x-y map, beam along +y) by gamma 1 %/1 mm and the
+    2-sigma fraction, R80 of the depth dose along the beam, the integral
+    dose and the depth dose within 3 combined errors."""
+    ref = np.load(GOLD_CT)
+    xe, ye, ze, hu, spots = ref["c3_xe"], ref["c3_ye"], ref["c3_ze"], ref["c3_hu"], ref["c3_spots"]
+    nz, ny, nx = hu.shape
+    ctx.set_grid_hu(xe, ye, ze, hu)
+    sp = spots.copy()
+    per_spot = 100000
+    sp["histories"] = per_spot
+    ctx.set_spots(sp)
+    ctx.set_scorers(1)
+    runs = []
+    for b in range(8):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=31, batched=True, hist_offset=b * per_spot))
+        assert res.status == 0 and res.histories == per_spot * len(sp)
+        runs.append(ctx.scorer(0).reshape(nz, ny, nx) / res.histories)
+    xy_g, xy_gs = _mean_se([r.sum(0) for r in runs])
+    xy_r, xy_rs = ref["c3_xy"].astype(np.float64), ref["c3_xy_se"].astype(np.float64)
+    rate, n = dosecmp.gamma_pass_rate(xy_r, xy_g, (float(ye[1] - ye[0]), float(xe[1] - xe[0])), dd=0.01, dta=1.0,
+                                      threshold=0.1, refine=2)
+    mask = xy_r >= 0.1 * xy_r.max()
+    frac = dosecmp.within_2sigma(xy_g, xy_gs, xy_r, xy_rs, mask)
+    want = dosecmp.two_sigma_expectation(xy_gs, xy_rs, 7, 7, mask)
+    print("x-y map: gamma %.4f of %d, within 2 sigma %.4f (equal physics: %.4f)" % (rate, n, frac, want))
+    depth = 0.5 * (ye[:-1] + ye[1:]).astype(np.float64)
+    idd_b = np.array([r.sum((0, 2)) for r in runs])
+    idd_g = idd_b.mean(0)
+    idd_gs = idd_b.std(0, ddof=1) / np.sqrt(len(idd_b))
+    r80_g, r80_r = dosecmp.r80(depth, idd_g), dosecmp.r80(depth, ref["c3_idd"])
+    ratio = idd_g.sum() / float(ref["c3_total"])
+    keep = ref["c3_idd"] >= 0.1 * ref["c3_idd"].max()
+    ok = np.abs(idd_g - ref["c3_idd"]) <= 3.0 * np.sqrt(idd_gs ** 2 + ref["c3_idd_se"] ** 2)
+    print("R80 gpu %.3f ref %.3f mm, integral ratio %.5f, depth dose within 3 sigma at %.3f of %d depths"
+          % (r80_g, r80_r, ratio, ok[keep].mean(), keep.sum()))
+    assert rate >= 0.99
+    assert frac >= want - 0.02
+    assert abs(r80_g - r80_r) <= 0.1
+    assert abs(ratio - 1.0) <= 0.005
+    assert ok[keep].mean() >= 0.9
