@@ -7,8 +7,9 @@ kernels):
 - the 2000-spot CT plan (512x512x200, 4.27e7 histories): dose is additive
   over spots (every history's random stream is keyed by seed, spot and
   history, so the plan's dose equals the sum of the doses of its even and odd
-  spots, up to FP64 summation order), and every history of the plan is
-  transported.
+  spots, up to FP64 summation order), every history of the plan is
+  transported, and the plain per-spot launches give the batched launch's
+  dose.
 """
 import numpy as np
 import pytest
@@ -67,3 +68,11 @@ def test_full_ct_plan_dose_is_additive_over_spots(ctx):
     full, even, odd = doses
     assert np.any(even != 0) and np.any(odd != 0)
     assert np.allclose(full, even + odd, rtol=1e-9, atol=1e-12 * full.max())
+    # the plain per-spot form (one launch per spot) transports the same
+    # histories: the dose the batched/plain ratio is quoted on is the same
+    ctx.set_spots(w.spots)
+    ctx.zero_scorers()
+    res = ctx.run(ctx.config(seed=13, batched=False))
+    assert res.status == 0 and res.histories == w.histories
+    plain = ctx.scorer(0)
+    assert np.allclose(plain, full, rtol=1e-9, atol=1e-12 * full.max())
