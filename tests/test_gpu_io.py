@@ -68,3 +68,74 @@ def test_roi_mask_files_score_like_their_sum(ctx, tmp_path):
         ctx.set_roi_mha(0, [str(tmp_path / "small.mha")])
     ctx.set_roi_mha(0)
     ctx.set_roi_mha(1)
+
+
+def test_ct_env_writes_what_the_abi_scores(ctx, tmp_path):
+    """mqi_hip::ct_env (tps_env without DICOM, moquimc_amd/host/mqi_hip_ct_env.hpp)
+    through a small driver: MetaImage CT, a scoring mask file, Dose + LETd
+    with the statistics pair (stopping rounds, saved), mhd output named
+    <beam>_<child>_<scorer> and scaled by particles per history x RBE x
+    fractions (1 x 1.1f x 1), equal to the ctypes path with the same seed;
+    a Dij run writes <beam>_0_Dij.npz equal to the ctypes CSR x 1.1f."""
+    import os
+    import subprocess
+    import scipy.sparse as sps
+    from moquimc_amd import _abi
+    root = rb.ROOT
+    exe = str(tmp_path / "ct_env_driver")
+    lib = os.path.join(root, "moquimc_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"), "-I", os.path.join(lib, "host"),
+                    os.path.join(root, "tests", "ct_env_driver.cpp"), "-L", lib, "-lmqi_hip", "-Wl,-rpath," + lib,
+                    "-o", exe], check=True)
+    rng = np.random.default_rng(9)
+    nz, ny, nx = 100, 24, 24
+    hu = rng.uniform(-150, 150, (nz, ny, nx)).astype(np.float32)
+    ct = tmp_path / "ct.mha"
+    _write_ct(ct, hu, (-23.0, -23.0, -199.0), (2.0, 2.0, 2.0))
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    mask = (((i - 11.5) ** 2 + (j - 11.5) ** 2 < 9.0) & (k >= 1)).astype(np.uint8)
+    mp = tmp_path / "mask.mha"
+    write_mask(mp, mask)
+    sp = np.array([rb.make_spot(110.0, 3.0, 400, pos=(-4.0, 0.0, 0.5)),
+                   rb.make_spot(130.0, 3.0, 300, pos=(4.0, 0.0, 0.5))], dtype=rb.SPOT_DTYPE)
+    spf = tmp_path / "spots.bin"
+    spf.write_bytes(sp.tobytes())
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([exe, str(ct), str(spf), str(out), "mhd", "7", "Dose,LETd", "1", str(mp)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert int(r.stdout.split()[-1]) >= 700
+    ctx.set_grid_ct_mha(str(ct))
+    ctx.set_spots(sp)
+    ctx.set_scorers(rb.SC_DOSE | rb.SC_LETD | rb.SC_STAT)
+    ctx.set_roi_mha(0, [str(mp)])
+    # tps_env's kernel with the statistics pair: transport_particles_patient_stat
+    ctx.run(ctx.config(seed=7, exact=True, batched=True, stat_criterion=0.5, stat_threshold=0.5,
+                       loops=_abi.LOOPS_REFERENCE_STAT))
+    names = ["Dose", "LETd_numer", "LETd_denom", "Dose_stat", "DoseSquare_stat"]
+    scale = float(np.float32(1.1))   # RBE is a float parameter, as the reference's
+    for kk, n in enumerate(names):
+        got = np.fromfile(out / ("beam_0_%s.raw" % n), np.float64)
+        want = ctx.scorer(kk) * scale
+        assert (out / ("beam_0_%s.mhd" % n)).exists()
+        assert np.allclose(got, want, rtol=1e-9, atol=0), n
+        assert np.any(got != 0), n
+    assert not np.any(np.fromfile(out / "beam_0_Dose.raw", np.float64)[mask.ravel() == 0])
+    ctx.set_roi_mha(0)
+    # Dij alone, npz
+    dout = tmp_path / "dij"
+    dout.mkdir()
+    r = subprocess.run([exe, str(ct), str(spf), str(dout), "raw", "7", "Dij", "0", "-"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ctx.set_scorers(rb.SC_DOSE)
+    ctx.run(ctx.config(seed=7, exact=True, batched=True, dij=True, dij_slots_sqrt=4096,
+                       loops=_abi.LOOPS_REFERENCE_PLAIN))
+    ctx.dij_build(0.0)
+    indptr, idx, val, shape = ctx.dij_csr()
+    D = sps.load_npz(dout / "beam_0_Dij.npz")
+    assert D.shape == shape
+    want = sps.csr_matrix((val.astype(np.float64) * scale, idx, indptr), shape=shape)
+    assert np.array_equal(D.indptr, want.indptr) and np.array_equal(D.indices, want.indices)
+    assert np.allclose(D.data, want.data, rtol=1e-6, atol=0)
