@@ -161,3 +161,23 @@ def test_host_driver_fails_cleanly_without_device():
     assert "mqi_create" in r.stderr
     r = subprocess.run([exe, "--nxyz", "4", "4"], capture_output=True, text=True)
     assert r.returncode == 1 and "--nxyz needs 3 values" in r.stderr
+
+
+@needs_lib
+def test_ct_env_driver_builds_and_fails_cleanly_without_device(tmp_path):
+    """The tps_env-style C++ environment (mqi_hip_ct_env.hpp) compiles and
+    links against the library; without a device, initialize() reports
+    mqi_create's error and the driver exits with status 1."""
+    import torch
+    exe = str(tmp_path / "ct_env_driver")
+    lib = os.path.join(ROOT, "moquimc_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+                    os.path.join(ROOT, "tests", "ct_env_driver.cpp"), "-L", lib, "-lmqi_hip", "-Wl,-rpath," + lib,
+                    "-o", exe], check=True)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    spots = tmp_path / "spots.bin"
+    spots.write_bytes(b"")
+    r = subprocess.run([exe, "/nonexistent.mha", str(spots), str(tmp_path), "raw", "1", "Dose", "0", "-"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.strip()
