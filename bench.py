@@ -62,9 +62,29 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers):
             "scorer_deposits_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1)}
 
 
+def _stdout_to_stderr(fn, *a):
+    """Run fn with file descriptor 1 on stderr (C stdio buffers flushed
+    before it is restored): the reference library prints to stdout, and the
+    one JSON line is the only thing bench.py writes there."""
+    import ctypes
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return fn(*a)
+    finally:
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
 def cpu_baseline(w, cores, target_s=20.0):
     """Reference CPU transport (oracle/_ref, release physics) on a bounded
     sample of the plan: every k-th spot with its full history count."""
+    return _stdout_to_stderr(_cpu_baseline, w, cores, target_s)
+
+
+def _cpu_baseline(w, cores, target_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import refbind as rb
     if not os.path.exists(os.path.join(rb.REF_DIR, "libmoqui_ref_release.so")):
