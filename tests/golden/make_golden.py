@@ -247,7 +247,7 @@ def profiles_fixture(threads, only=None):
     return out
 
 
-def ct_fixture(threads, per_spot=10000, n_batches=8):
+def ct_fixture(threads, per_spot=10000, n_batches=8, let_per_spot=0, let_batches=0):
     """The 100-spot sub-plan on the cropped CT of the 2000-spot plan
     (phantoms.ct_crop_plan), release physics, n_batches x per_spot histories
     per spot.  Vertices from the beamlet sampling of the spot records
@@ -257,7 +257,8 @@ def ct_fixture(threads, per_spot=10000, n_batches=8):
     the reference's hu_to_density.  Stored: the inputs (HU crop, edges,
     spots) and the dose summed over z (x-y map), its integral depth dose
     along y and the integral dose, with batch standard errors (Gy per
-    history)."""
+    history); with let_batches, the per-batch LETd numerator and denominator
+    summed per depth."""
     t0 = time.time()
     xe, ye, ze, hu, spots = phantoms.ct_crop_plan(per_spot)
     nz, ny, nx = hu.shape
@@ -278,6 +279,26 @@ def ct_fixture(threads, per_spot=10000, n_batches=8):
         print("  batch %d: %d histories, %.0f s" % (b, tracked, time.time() - t0), flush=True)
     out = {"c3_xe": xe, "c3_ye": ye, "c3_ze": ze, "c3_hu": hu, "c3_spots": spots,
            "c3_histories": np.array(n_batches * per_spot * len(spots))}
+    if let_batches:
+        # LETd numerator and denominator (scored without the dose scorer): per-batch
+        # sums per depth along the beam (y)
+        lets = []
+        for b in range(let_batches):
+            vtx, seeds = [], []
+            for k, sp in enumerate(spots):
+                v, _ = rb.oracle_sample_spot(sp, 5000 + b, k, let_per_spot, rng=0)
+                vtx.append(v)
+                seeds.append(np.random.default_rng([5000 + b, k]).integers(1, 2 ** 31 - 1,
+                                                                            size=let_per_spot).astype(np.int32))
+            arrs, tracked = rb.ref_transport(0, g, np.concatenate(vtx), np.concatenate(seeds), mask=rb.SC_LETD,
+                                             kernel=0, threads=threads)
+            lets.append([a.reshape(nz, ny, nx).sum((0, 2)) for a in arrs])
+            print("  LETd batch %d: %d histories, %.0f s" % (b, tracked, time.time() - t0), flush=True)
+        out["c3_let_num_y"] = np.array([x[0] for x in lets])
+        out["c3_let_den_y"] = np.array([x[1] for x in lets])
+        out["c3_let_histories"] = np.array(let_batches * let_per_spot * len(spots))
+        if not n_batches:
+            return out
     out["c3_xy"], out["c3_xy_se"] = _mean_se([r.sum(0) for r in runs])
     out["c3_idd"], out["c3_idd_se"] = _mean_se([r.sum((0, 2)) for r in runs])
     out["c3_total"], out["c3_total_se"] = _mean_se([r.sum() for r in runs])
@@ -291,13 +312,21 @@ def main():
     ap.add_argument("--only", choices=["c2"], default=None, help="with --stats: recompute the slab workload only")
     ap.add_argument("--ct", action="store_true", help="write reference_ct.npz (the cropped-CT sub-plan)")
     ap.add_argument("--ct-per-spot", type=int, default=10000)
+    ap.add_argument("--ct-let-only", action="store_true", help="with --ct: add the LETd arrays to reference_ct.npz")
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
     args = ap.parse_args()
     if not rb.have_reference():
         raise SystemExit("build the reference harness first: make -C oracle")
     if args.ct:
-        out = ct_fixture(args.threads, per_spot=args.ct_per_spot)
-        np.savez_compressed(os.path.join(HERE, "reference_ct.npz"), **out)
+        path = os.path.join(HERE, "reference_ct.npz")
+        if args.ct_let_only:
+            out = dict(np.load(path))
+            new = ct_fixture(args.threads, per_spot=args.ct_per_spot, n_batches=0, let_per_spot=args.ct_per_spot,
+                             let_batches=4)
+            out.update({k: v for k, v in new.items() if k.startswith("c3_let")})
+        else:
+            out = ct_fixture(args.threads, per_spot=args.ct_per_spot)
+        np.savez_compressed(path, **out)
         return
     if args.stats:
         out = profiles_fixture(args.threads, only=args.only)

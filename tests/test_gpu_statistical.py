@@ -246,3 +246,45 @@ def test_ct_subplan_against_reference(ctx):
     assert abs(r80_g - r80_r) <= 0.1
     assert abs(ratio - 1.0) <= 0.005
     assert ok[keep].mean() >= 0.9
+
+
+def _has_ct_let():
+    return os.path.exists(GOLD_CT) and "c3_let_num_y" in np.load(GOLD_CT).files
+
+
+@pytest.mark.skipif(not _has_ct_let(), reason="reference_ct.npz has no LETd arrays")
+def test_ct_subplan_letd_against_reference(ctx):
+    """Dose-averaged LET per depth along the beam over the whole cropped CT
+    for the 100-spot sub-plan (reference 4 batches x 1e4 histories per spot,
+    LETd scored without the dose scorer): equal within 3 combined batch
+    standard errors at 90 % of the depths above 10 % of the maximum, median
+    relative difference below 1 %."""
+    ref = np.load(GOLD_CT)
+    xe, ye, ze, hu, spots = ref["c3_xe"], ref["c3_ye"], ref["c3_ze"], ref["c3_hu"], ref["c3_spots"]
+    nz, ny, nx = hu.shape
+    ctx.set_grid_hu(xe, ye, ze, hu)
+    sp = spots.copy()
+    per_spot = 50000
+    sp["histories"] = per_spot
+    ctx.set_spots(sp)
+    ctx.set_scorers(2)   # LETd numerator and denominator
+    g_num, g_den = [], []
+    for b in range(4):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=33, batched=True, hist_offset=b * per_spot))
+        assert res.status == 0 and res.histories == per_spot * len(sp)
+        g_num.append(ctx.scorer(0).reshape(nz, ny, nx).sum((0, 2)))
+        g_den.append(ctx.scorer(1).reshape(nz, ny, nx).sum((0, 2)))
+    g_num, g_den = np.array(g_num), np.array(g_den)
+    r_num, r_den = ref["c3_let_num_y"], ref["c3_let_den_y"]
+    keep = r_den.sum(0) >= 0.1 * r_den.sum(0).max()
+    lg = g_num[:, keep].sum(0) / g_den[:, keep].sum(0)
+    lr = r_num[:, keep].sum(0) / r_den[:, keep].sum(0)
+    sg = (g_num[:, keep] / g_den[:, keep]).std(0, ddof=1) / 2.0
+    sr = (r_num[:, keep] / r_den[:, keep]).std(0, ddof=1) / 2.0
+    within = np.abs(lg - lr) <= 3.0 * np.sqrt(sg ** 2 + sr ** 2)
+    rel = np.abs(lg / lr - 1)
+    print("CT LETd per depth: %d depths, within 3 sigma %.3f, median rel diff %.4f, max %.4f" % (
+        keep.sum(), within.mean(), np.median(rel), rel.max()))
+    assert within.mean() >= 0.9
+    assert np.median(rel) <= 0.01
