@@ -341,3 +341,37 @@ def test_exact_roi_masks_match_oracle(ctx, debug):
     indptr, idx, _, _ = ctx.dij_csr()
     assert indptr[-1] > 0 and np.all(in0[idx[:indptr[-1]]])
     ctx.set_roi(0)
+
+
+@pytest.mark.gpu
+def test_exact_robust_scenarios_match_oracle_on_shifted_and_scaled_grids(ctx):
+    """Robust scenarios in one launch (mqi_set_scenarios): the verification
+    kernels' dose of a scenario with an edge shift equals the oracle's on the
+    grid whose edges are shifted by it (float32 e + s), and of a scenario with
+    a density scale the oracle's on the density times it (float32), each with
+    the scenario's own random streams; up to FP64 summation order."""
+    g = phantoms.slab_box(30, 30, 200)
+    sp = phantoms.spots([130.0, 160.0], 250, sigma=4.0, sigma_e_frac=0.01, positions=[(0, 0, 0.5), (3, -2, 0.5)])
+    shifts = np.array([[0, 0, 0], [1.3, -0.7, 2.1], [0, 0, 0]], np.float32)
+    scales = np.array([1.0, 1.0, 1.035], np.float32)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scenarios(shifts, scales)
+    ctx.set_scorers(rb.SC_DOSE | rb.SC_LETD, per_scenario=True)
+    res = ctx.run(ctx.config(seed=37, exact=True, batched=True))
+    assert res.histories == 3 * 500
+    grids = [g,
+             rb.Grid(g.xe + shifts[1, 0], g.ye + shifts[1, 1], g.ze + shifts[1, 2], g.density),
+             rb.Grid(g.xe, g.ye, g.ze, g.density * scales[2])]
+    for s, gs in enumerate(grids):
+        ref, ntr = rb.oracle_run_spots(gs, sp, mask=rb.SC_DOSE | rb.SC_LETD, kernel=2, prec=32, rng=1, math=1,
+                                       debug=0, master_seed=37, threads=4, scenario=s)
+        assert ntr == 500
+        for k in range(3):
+            got, want = ctx.scorer(k, s), ref[k]
+            assert np.allclose(got, want, rtol=1e-9, atol=0), (s, k)
+            assert np.any(want != 0)
+    # the scenarios differ from each other
+    assert not np.allclose(ctx.scorer(0, 1), ctx.scorer(0, 0))
+    assert not np.allclose(ctx.scorer(0, 2), ctx.scorer(0, 0))
+    ctx.set_scenarios()
