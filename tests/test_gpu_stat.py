@@ -73,3 +73,29 @@ def test_stopping_reaches_criterion(ctx):
     res = ctx.run(ctx.config(seed=33, batched=True, stat_criterion=0.2, stat_threshold=0.5, max_rounds=20))
     assert res.stat <= 0.2
     assert 1 <= res.rounds < 20
+
+
+def test_stat_over_a_statistics_roi(ctx):
+    """With a statistics ROI the reference keeps the pair compactly over the
+    ROI's voxels and calculate_stat takes the maximum and the mean over those
+    (mqi_tps_env.hpp:797-815,1400-1426); here the pair is dense with zeros off
+    the ROI: the statistic equals the closed form on the ROI's voxels alone,
+    and the stopping loop runs on it."""
+    g, sp = _setup(ctx)
+    k, j, i = np.meshgrid(np.arange(140), np.arange(24), np.arange(24), indexing="ij")
+    roi = (((i - 12) ** 2 + (j - 12) ** 2 < 9) & (k >= 60) & (k < 130)).astype(np.uint8).ravel()
+    ctx.set_roi(1, roi)
+    try:
+        res = ctx.run(ctx.config(seed=34, batched=True))
+        s1, s2 = ctx.scorer(1), ctx.scorer(2)
+        assert not np.any(s1[roi == 0]) and not np.any(s2[roi == 0])
+        inside = roi == 1
+        for thr in (0.5, 0.0):
+            want = _numpy_stat(s1[inside], s2[inside], res.histories, thr)
+            got = ctx.stat(res.histories, thr)
+            assert abs(got - want) <= 1e-9 * want, thr
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=35, batched=True, stat_criterion=0.2, stat_threshold=0.5, max_rounds=8))
+        assert res.stat <= 0.2 or res.rounds == 8
+    finally:
+        ctx.set_roi(1)
