@@ -59,7 +59,9 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers):
             "traffic_source": TRAFFIC_SOURCE if m else None,
             "kernel": "k_transport<release, dynamic> (persistent)", "launch_ms": round(ms_per_launch, 3),
             "voxel_steps_per_launch": int(steps / launches), "bytes_per_step": bytes_per_step,
-            "scorer_deposits_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1)}
+            "scorer_deposits_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1),
+            "limiter": "VALU issue (~88% of cycles, SQ counters) and the FP64 atomic stream; neither HBM nor "
+                       "MFMA bounds this pass (profiles/hbm_counters_ct2000_dij5000.md)"}
 
 
 def _stdout_to_stderr(fn, *a):
