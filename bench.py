@@ -64,6 +64,40 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers):
                        "MFMA bounds this pass (profiles/hbm_counters_ct2000_dij5000.md)"}
 
 
+def dose_agreement(batched, plain, w, dd=0.01, dta=1.0, threshold=0.1):
+    """Agreement of the batched and the plain form's dose on the same
+    histories: the largest difference relative to the maximum, and the
+    global 3-D gamma pass rate (dd of the maximum, dta mm, voxels above
+    threshold of the maximum; the evaluated dose searched over the voxel
+    offsets within dta, no interpolation), computed on the GPU."""
+    import itertools
+
+    import torch
+    dev = torch.device("cuda", 0)
+    nz, ny, nx = w.shape
+    spacing = (float(w.ze[1] - w.ze[0]), float(w.ye[1] - w.ye[0]), float(w.xe[1] - w.xe[0]))
+    r = torch.from_numpy(batched.reshape(nz, ny, nx)).to(dev)
+    e = torch.from_numpy(plain.reshape(nz, ny, nx)).to(dev)
+    dmax = r.max()
+    reach = [int(dta // sp) for sp in spacing]
+    ep = torch.nn.functional.pad(e[None, None], (reach[2], reach[2], reach[1], reach[1], reach[0], reach[0]),
+                                 value=float("inf"))[0, 0]
+    best = torch.full_like(r, float("inf"))
+    for off in itertools.product(*[range(-k, k + 1) for k in reach]):
+        d2 = sum((o * sp) ** 2 for o, sp in zip(off, spacing)) / (dta * dta)
+        if d2 > 1.0:
+            continue
+        sh = ep[reach[0] + off[0]:reach[0] + off[0] + nz, reach[1] + off[1]:reach[1] + off[1] + ny,
+                reach[2] + off[2]:reach[2] + off[2] + nx]
+        best = torch.minimum(best, d2 + ((sh - r) / (dd * dmax)) ** 2)
+    mask = r >= threshold * dmax
+    out = {"gamma_1pct_1mm": round(float((best[mask] <= 1.0).double().mean()), 6), "voxels": int(mask.sum()),
+           "max_abs_diff_of_max": float("%.3g" % float((r - e).abs().max() / dmax))}
+    del r, e, ep, best, mask
+    torch.cuda.empty_cache()
+    return out
+
+
 def _stdout_to_stderr(fn, *a):
     """Run fn with file descriptor 1 on stderr (C stdio buffers flushed
     before it is restored): the reference library prints to stdout, and the
@@ -80,9 +114,22 @@ def _stdout_to_stderr(fn, *a):
         os.close(saved)
 
 
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(w, cores, target_s=20.0):
     """Reference CPU transport (oracle/_ref, release physics) on a bounded
-    sample of the plan: every k-th spot with its full history count."""
+    sample of the plan: every k-th spot with its full history count, on all
+    the cores this process may run on, plus a one-core rate."""
     return _stdout_to_stderr(_cpu_baseline, w, cores, target_s)
 
 
@@ -119,11 +166,17 @@ def _cpu_baseline(w, cores, target_s):
     n = int(min(len(seeds), max(pilot, rate * target_s)))
     t = []
     _, tracked = rb.ref_transport(0, grid, vtx[:n], seeds[:n], mask=mask, kernel=1, threads=cores, timing=t)
+    # one core, on the first histories of the same sample
+    n1 = int(min(n, max(2000, rate / cores * 5.0)))
+    t1 = []
+    _, tracked1 = rb.ref_transport(0, grid, vtx[:n1], seeds[:n1], mask=mask, kernel=1, threads=1, timing=t1)
     return {"value": round(tracked / t[0], 1), "unit": "histories/s", "cores": cores, "kind": "reference",
+            "cpu_model": cpu_model(), "per_core_histories_per_s": round(tracked1 / t1[0], 1),
             "sample": "%d histories of %d spots (every %dth spot of the plan, full history counts), reference "
                       "transport_particles_patient_stat on the CPU (dose + stat scorers), release physics, "
-                      "%d threads, %.1f s"
-                      % (tracked, len(picks), stride, cores, t[0])}
+                      "%d threads with a world and scorer tables each (independent, as separate processes), "
+                      "%.1f s; one-core rate from %d of those histories in %.1f s"
+                      % (tracked, len(picks), stride, cores, t[0], tracked1, t1[0])}
 
 
 def adjoint_bench(ctx_cls, reps=20):
@@ -301,6 +354,7 @@ def main():
     ap.add_argument("--robust-only", action="store_true", help="only the 9-scenario robust Dij leg")
     ap.add_argument("--exact", action="store_true", help="time the verification kernels instead")
     ap.add_argument("--no-tune", action="store_true", help="default launch parameters instead of the autotuner's")
+    ap.add_argument("--no-graph", action="store_true", help="launch the batched rounds directly, not from a hipGraph")
     ap.add_argument("--adjoint-only", action="store_true", help="only the Dij workload and its products")
     ap.add_argument("--no-phantoms", action="store_true", help="skip the water and slab phantom leg")
     ap.add_argument("--phantoms-only", action="store_true", help="only the water and slab phantom leg")
@@ -340,12 +394,15 @@ def main():
     stat = dict(stat_criterion=0.01, stat_threshold=0.5, max_rounds=16)
     cfg = ctx.config(seed=11, batched=True, exact=args.exact, **stat)
     tuned = None
+    dropped = 0
     if not args.exact and not args.no_tune:
         t0 = time.time()
         cfg, best = ctx.autotune(cfg, sample=0.1)
         tuned = {"waves_per_simd": cfg.waves_per_simd, "block_size": cfg.block_size, "fetch_chunk": cfg.fetch_chunk,
                  "refill_lanes": cfg.refill_lanes, "autotune_s": round(time.time() - t0, 2)}
         log("autotuned: %s" % tuned)
+    # the batched round is captured in a hipGraph once and replayed per round
+    cfg.use_graph = 0 if args.no_graph else 1
     for i in range(args.warmup):
         ctx.zero_scorers()
         r = ctx.run(cfg)
@@ -361,22 +418,33 @@ def main():
         hist += last.histories
         launches += last.rounds
         steps += last.steps
+        dropped += last.dropped
         log("step %d: %.3f s kernel, %d histories, %d rounds, stat %.4f, %d voxel steps" % (
             i, last.kernel_ms / 1e3, last.histories, last.rounds, last.stat, last.steps))
     wall = time.perf_counter() - tw0
     ms_per_step = wall / args.steps * 1e3
     value = hist / wall
 
+    # the dose of one batched pass (untimed) on the histories of the first plain pass
+    ctx.zero_scorers()
+    cfg.hist_offset = 1 << 24
+    ctx.run(cfg)
+    dose_batched = ctx.scorer(0)
+
     # plain per-spot launches (one launch and host sync per spot, as run_by_spot), same criterion
-    plain_ms, plain_hist = 0.0, 0
-    tp0 = time.perf_counter()
+    plain_ms, plain_hist, plain_wall, agree = 0.0, 0, 0.0, None
     for i in range(args.plain_passes):
         ctx.zero_scorers()
+        tp0 = time.perf_counter()
         r = ctx.run(ctx.config(seed=11, batched=False, exact=args.exact, hist_offset=(i + 1) * (1 << 24), **stat))
+        plain_wall += time.perf_counter() - tp0
         plain_ms += r.kernel_ms
         plain_hist += r.histories
         log("plain pass %d: %.3f s kernel, %d rounds" % (i, r.kernel_ms / 1e3, r.rounds))
-    plain_wall = time.perf_counter() - tp0
+        if i == 0:
+            agree = dose_agreement(dose_batched, ctx.scorer(0), w)
+            log("batched vs plain dose: %s" % agree)
+    del dose_batched
     plain_rate = plain_hist / plain_wall if plain_hist else None
 
     out = {"metric": METRIC, "value": round(value, 1), "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
@@ -386,12 +454,15 @@ def main():
            "config": {"workload": w.name + ", one persistent launch per round, stat stop 1% above 50% max, "
                                            "release physics",
                       "histories_per_round": int(w.histories), "spots": int(len(w.spots)),
-                      "grid": "512x512x200", "scorers": "dose, dose_stat, dose_square_stat", "tuned": tuned},
+                      "grid": "512x512x200", "scorers": "dose, dose_stat, dose_square_stat", "tuned": tuned,
+                      "hip_graph": bool(cfg.use_graph), "stack_depth": "10 (the reference's release depth)"},
            "rounds_per_step": round(launches / args.steps, 2),
            "kernel_histories_per_s": round(hist / (kernel_ms / 1e3), 1),
            "plain_histories_per_s": round(plain_rate, 1) if plain_rate else None,
            "plain_kernel_histories_per_s": round(plain_hist / (plain_ms / 1e3), 1) if plain_ms else None,
            "batched_over_plain": round(value / plain_rate, 3) if plain_rate else None,
+           "batched_vs_plain_dose": agree,
+           "secondaries_dropped_per_step": round(dropped / args.steps, 1),
            "steps_per_history": round(steps / max(1, hist), 1),
            "roofline": transport_roofline(steps, kernel_ms, launches, n_scorers)}
     ctx.close()
@@ -411,7 +482,7 @@ def main():
         except Exception as e:  # reported, not hidden
             out["phantoms_error"] = repr(e)
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
+        cores = len(os.sched_getaffinity(0))
         try:
             out["cpu_baseline"] = cpu_baseline(w, cores)
         except Exception as e:

@@ -277,6 +277,22 @@ int mqi_last_matvec_ms(mqi_ctx* ctx, double* ms);
  * 9 float sqrt) on n host inputs. */
 int mqi_math_probe(mqi_ctx* ctx, int fn, const double* x, const double* y, int64_t n, double* out);
 
+/* Verification: the material and physics functions as the transport kernels
+ * evaluate them on the device, in the verification (exact = 1) or production
+ * (exact = 0) build, with phantom-build (physics_debug = 1) or release
+ * physics, on n host inputs:
+ *   fn 0  spr_default(rho[i], ek[i]) (mqi_patient_materials.hpp:415-450), n floats out;
+ *   fn 1  radiation_length_default(rho[i]) (:452-473; ek may be NULL), n floats out;
+ *   fn 2  per i, 10 floats: the cross sections of ionization, p-p elastic,
+ *         p-O elastic and p-O inelastic at (ek[i], rho[i])
+ *         (mqi_p_ionization.hpp:254-268, mqi_pp_elastic.hpp:221-235,
+ *         mqi_po_elastic.hpp, mqi_po_inelastic.hpp), dEdx
+ *         (mqi_p_ionization.hpp:271-286), and beta^2, gamma, Et, Te_max,
+ *         momentum (mqi_relativistic_quantities.hpp:24-54).
+ * rho in g/mm^3, ek in MeV. */
+int mqi_physics_probe(mqi_ctx* ctx, int exact, int physics_debug, int fn, const float* rho, const float* ek, int64_t n,
+                      float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,7 @@ def lib():
             "mqi_last_matvec_ms": (i, [vp, ctypes.POINTER(d)]),
             "mqi_autotune": (i, [vp, ctypes.POINTER(RunConfig), d, ctypes.POINTER(d)]),
             "mqi_math_probe": (i, [vp, i, vp, vp, ctypes.c_int64, vp]),
+            "mqi_physics_probe": (i, [vp, i, i, i, vp, vp, ctypes.c_int64, vp]),
             "mqi_dij_save_npz": (i, [vp, i, ctypes.c_char_p, d]),
             "mqi_set_grid_ct_mha": (i, [vp, ctypes.c_char_p, f, vp]),
             "mqi_set_beamline": (i, [vp, i, vp]),
@@ -106,7 +107,8 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
-            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha"]
+            "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha",
+            "mqi_physics_probe"]
 
 
 class GridDesc(ctypes.Structure):
@@ -372,4 +374,14 @@ class Context:
         y = x if y is None else np.ascontiguousarray(y, np.float64)
         out = np.zeros_like(x)
         self._chk(self._L.mqi_math_probe(self.h, fn, _p(x), _p(y), len(x), _p(out)), "mqi_math_probe")
+        return out
+
+    def physics_probe(self, fn, rho, ek=None, exact=True, physics_debug=False):
+        """fn 0: spr(rho, ek); 1: radiation length(rho); 2: (n, 10) cross
+        sections, dEdx and relativistic quantities (mqi_physics_probe)."""
+        rho = np.ascontiguousarray(rho, np.float32)
+        ek = rho if ek is None else np.ascontiguousarray(ek, np.float32)
+        out = np.zeros((rho.size, 10) if fn == 2 else rho.size, np.float32)
+        self._chk(self._L.mqi_physics_probe(self.h, int(exact), int(physics_debug), fn, _p(rho), _p(ek), rho.size,
+                                            _p(out)), "mqi_physics_probe")
         return out

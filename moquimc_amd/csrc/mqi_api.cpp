@@ -19,6 +19,7 @@
 #include "../../include/mqi_hip.h"
 #include "mqi_aux.h"
 #include "mqi_kernels.h"
+#include "mqi_limits.h"
 #include "../host/mqi_hip_io.hpp"
 
 namespace
@@ -1295,6 +1296,10 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
         std::vector<int64_t> indptr(n_spots + 1, 0);
         for (int j = 0; j < n_spots; ++j) indptr[j + 1] = indptr[j] + cnt[(size_t) s * n_spots + j];
         m.nnz = indptr[n_spots];
+        if ((r = dij_size_check(m.nnz)))
+            return fail(c, r, "scenario %d's Dij has %lld nonzeros: its rows are sorted with 32-bit counts "
+                              "(at most %lld); raise the threshold or split the plan", s, (long long) m.nnz,
+                        (long long) MQI_MAX_SORT_NNZ);
         if ((r = dalloc(c, &m.indptr, n_spots + 1)) || (r = dalloc(c, &m.col, std::max<int64_t>(1, m.nnz))) ||
             (r = dalloc(c, &m.val, std::max<int64_t>(1, m.nnz))))
             return r;
@@ -1318,12 +1323,11 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             hipFree(scratch);
             hipFree(tk);
             hipFree(tv);
-            // voxel-major copy for D w (counting sort by voxel)
-            uint32_t*           d_cc  = nullptr;
-            unsigned long long* d_cur = nullptr;
-            if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &d_cur, m.cols)) ||
-                (r = dalloc(c, &m.tindptr, m.cols + 1)) || (r = dalloc(c, &m.tcol, m.nnz)) ||
-                (r = dalloc(c, &m.tval, m.nnz)))
+            // voxel-major copy for D w: row lengths by counting, entries by a
+            // stable sort on the voxel (each voxel's spots ascending)
+            uint32_t* d_cc = nullptr;
+            if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &m.tindptr, m.cols + 1)) ||
+                (r = dalloc(c, &m.tcol, m.nnz)) || (r = dalloc(c, &m.tval, m.nnz)))
                 return r;
             KCHK(c, mqi_aux::fill_u32(d_cc, 0u, m.cols, c->stream));
             KCHK(c, mqi_aux::csr_transpose_count(m.col, m.nnz, d_cc, c->stream));
@@ -1350,11 +1354,25 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             KCHK(c, mqi_aux::renumber(m.col, m.nnz, d_rank, m.ccol, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             hipFree(d_rank);
-            HIPCHK(c, hipMemcpy(d_cur, tptr.data(), sizeof(int64_t) * m.cols, hipMemcpyHostToDevice));
-            KCHK(c, mqi_aux::csr_transpose_scatter(m.indptr, m.col, m.val, m.rows, d_cur, m.tcol, m.tval, c->stream));
+            int key_bits = 1;
+            while (key_bits < 32 && (1ll << key_bits) < m.cols) ++key_bits;
+            int32_t*  d_keys = nullptr;
+            uint64_t *d_pk = nullptr, *d_pk2 = nullptr;
+            char*     d_tmp = nullptr;
+            size_t    tbytes = 0;
+            KCHK(c, mqi_aux::csr_transpose_sorted(m.indptr, m.col, m.val, m.rows, m.nnz, key_bits, nullptr, nullptr,
+                                                  nullptr, nullptr, nullptr, nullptr, &tbytes, c->stream));
+            if ((r = dalloc(c, &d_keys, m.nnz)) || (r = dalloc(c, &d_pk, m.nnz)) || (r = dalloc(c, &d_pk2, m.nnz)) ||
+                (r = dalloc(c, &d_tmp, std::max<size_t>(tbytes, 1))))
+                return r;
+            KCHK(c, mqi_aux::csr_transpose_sorted(m.indptr, m.col, m.val, m.rows, m.nnz, key_bits, d_keys, d_pk, d_pk2,
+                                                  m.tcol, m.tval, d_tmp, &tbytes, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+            hipFree(d_keys);
+            hipFree(d_pk);
+            hipFree(d_pk2);
+            hipFree(d_tmp);
             hipFree(d_cc);
-            hipFree(d_cur);
         }
     }
     hipFree(d_max);
@@ -1610,6 +1628,34 @@ mqi_math_probe(mqi_ctx* c, int fn, const double* x, const double* y, int64_t n, 
     HIPCHK(c, hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost));
     hipFree(dx);
     hipFree(dy);
+    hipFree(dout);
+    return 0;
+}
+
+int
+mqi_physics_probe(mqi_ctx* c, int exact, int physics_debug, int fn, const float* rho, const float* ek, int64_t n,
+                  float* out) {
+    if (!c || !rho || !out || n <= 0 || fn < 0 || fn > 2) return MQI_E_ARG;
+    if (fn != 1 && !ek) return MQI_E_ARG;
+    hipSetDevice(c->device);
+    const int64_t n_out = fn == 2 ? 10 * n : n;
+    float *       drho = nullptr, *dek = nullptr, *dout = nullptr;
+    int           r;
+    if ((r = dalloc(c, &drho, n)) || (r = dalloc(c, &dek, n)) || (r = dalloc(c, &dout, n_out))) {
+        hipFree(drho);
+        hipFree(dek);
+        return r;
+    }
+    HIPCHK(c, hipMemcpy(drho, rho, sizeof(float) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dek, ek ? ek : rho, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (exact)
+        KCHK(c, mqi_exact::physics_probe(physics_debug, fn, drho, dek, n, dout, c->stream));
+    else
+        KCHK(c, mqi_fast::physics_probe(physics_debug, fn, drho, dek, n, dout, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, dout, sizeof(float) * n_out, hipMemcpyDeviceToHost));
+    hipFree(drho);
+    hipFree(dek);
     hipFree(dout);
     return 0;
 }

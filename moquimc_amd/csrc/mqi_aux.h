@@ -40,8 +40,13 @@ int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, co
 int gather_x(const float* x, const int32_t* nzvox, int64_t n, float* xc, hipStream_t s);
 int renumber(const int32_t* col, int64_t nnz, const int32_t* rank, int32_t* ccol, hipStream_t s);
 int csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t s);
-int csr_transpose_scatter(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
-                          unsigned long long* cursor, int32_t* tcol, float* tval, hipStream_t s);
+// voxel-major copy (tcol = spot, tval) with each voxel's spots ascending:
+// stable radix sort of the nonzeros by voxel (key_bits bits); with scratch
+// NULL only *scratch_bytes is set.  keys_tmp: nnz ints, packed / packed_tmp:
+// nnz 64-bit words.
+int csr_transpose_sorted(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows, int64_t nnz,
+                         int key_bits, int32_t* keys_tmp, uint64_t* packed, uint64_t* packed_tmp, int32_t* tcol,
+                         float* tval, void* scratch, size_t* scratch_bytes, hipStream_t s);
 }   // namespace mqi_aux
 
 #endif

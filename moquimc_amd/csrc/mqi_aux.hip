@@ -464,34 +464,56 @@ renumber(const int32_t* col, int64_t nnz, const int32_t* rank, int32_t* ccol, hi
     return (int) hipGetLastError();
 }
 
-// Transpose of a CSR matrix by counting sort: voxel-major copy for D w.
+// Transpose of a CSR matrix: voxel-major copy for D w (row lengths by
+// counting, then the entries in a fixed order).
 __global__ void k_count_cols(const int32_t* col, int64_t nnz, uint32_t* cnt) {
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t) gridDim.x * blockDim.x)
         atomicAdd(cnt + col[i], 1u);
 }
-__global__ void k_scatter_t(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
-                            unsigned long long* cursor, int32_t* tcol, float* tval) {
+// The voxel-major copy lists each voxel's spots in ascending order whatever
+// the scheduling: every nonzero becomes (voxel; spot, value) with the spot and
+// the value's bits packed in 64 bits, and a stable LSD radix sort on the voxel
+// keeps the spot-major order (spots ascending) inside each voxel.
+__global__ void k_pack_spot_value(const int64_t* indptr, const float* val, int64_t n_rows, uint64_t* packed) {
     const int     lane  = threadIdx.x & 63;
     const int64_t wave  = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t waves = ((int64_t) gridDim.x * blockDim.x) >> 6;
     for (int64_t r = wave; r < n_rows; r += waves)
-        for (int64_t k = indptr[r] + lane; k < indptr[r + 1]; k += 64) {
-            unsigned long long p = atomicAdd(cursor + col[k], 1ull);
-            tcol[p]              = (int32_t) r;
-            tval[p]              = val[k];
-        }
+        for (int64_t k = indptr[r] + lane; k < indptr[r + 1]; k += 64)
+            packed[k] = ((uint64_t) (uint32_t) r << 32) | (uint64_t) __float_as_uint(val[k]);
+}
+__global__ void k_unpack_spot_value(const uint64_t* packed, int64_t nnz, int32_t* tcol, float* tval) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t) gridDim.x * blockDim.x) {
+        const uint64_t p = packed[i];
+        tcol[i]          = (int32_t) (uint32_t) (p >> 32);
+        tval[i]          = __uint_as_float((uint32_t) p);
+    }
+}
+
+int
+csr_transpose_sorted(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows, int64_t nnz,
+                     int key_bits, int32_t* keys_tmp, uint64_t* packed, uint64_t* packed_tmp, int32_t* tcol,
+                     float* tval, void* scratch, size_t* scratch_bytes, hipStream_t s) {
+    if (!scratch) {
+        return (int) hipcub::DeviceRadixSort::SortPairs(nullptr, *scratch_bytes, (const uint32_t*) col,
+                                                        (uint32_t*) keys_tmp, packed, packed_tmp, (size_t) nnz, 0,
+                                                        key_bits, s);
+    }
+    if (nnz <= 0) return 0;
+    hipLaunchKernelGGL(k_pack_spot_value, dim3(4096), dim3(256), 0, s, indptr, val, n_rows, packed);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(scratch, *scratch_bytes, (const uint32_t*) col,
+                                                      (uint32_t*) keys_tmp, (const uint64_t*) packed, packed_tmp,
+                                                      (size_t) nnz, 0, key_bits, s);
+    if (e != hipSuccess) return (int) e;
+    hipLaunchKernelGGL(k_unpack_spot_value, dim3(grid_for((uint64_t) nnz)), dim3(256), 0, s, packed_tmp, nnz, tcol,
+                       tval);
+    return (int) hipGetLastError();
 }
 
 int
 csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t s) {
     if (nnz <= 0) return 0;
     hipLaunchKernelGGL(k_count_cols, dim3(grid_for((uint64_t) nnz)), dim3(256), 0, s, col, nnz, cnt);
-    return (int) hipGetLastError();
-}
-int
-csr_transpose_scatter(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows,
-                      unsigned long long* cursor, int32_t* tcol, float* tval, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter_t, dim3(4096), dim3(256), 0, s, indptr, col, val, n_rows, cursor, tcol, tval);
     return (int) hipGetLastError();
 }
 

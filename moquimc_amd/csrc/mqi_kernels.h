@@ -123,12 +123,14 @@ namespace mqi_fast
 int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int waves, int blocks,
                      int threads, size_t shmem, hipStream_t stream);
 int occupancy(int debug, int dynamic, int trace, int waves, int threads, size_t shmem);
+int physics_probe(int debug, int fn, const float* rho, const float* ek, int64_t n, float* out, hipStream_t stream);
 }
 namespace mqi_exact
 {
 int launch_transport(const transport_params& P, int debug, int dynamic, int trace, int waves, int blocks,
                      int threads, size_t shmem, hipStream_t stream);
 int occupancy(int debug, int dynamic, int trace, int waves, int threads, size_t shmem);
+int physics_probe(int debug, int fn, const float* rho, const float* ek, int64_t n, float* out, hipStream_t stream);
 }
 
 #endif

@@ -1152,6 +1152,57 @@ launch_transport(const transport_params& P, int debug, int dynamic, int trace, i
     return (int) hipGetLastError();
 }
 
+// Verification probes of the material and physics functions as the
+// transport kernels evaluate them: fn 0 spr (rho, ek), fn 1 radiation length
+// (rho), fn 2 per (rho, ek) the four cross sections at rho, dEdx and the
+// relativistic quantities beta^2, gamma, Et, Te_max, momentum (10 floats).
+template<bool DBG>
+__global__ void __launch_bounds__(256) k_physics_probe(int fn, const float* rho, const float* ek, int64_t n, float* out) {
+    __shared__ float sh[T_TABLES];
+    for (int i = threadIdx.x; i < 600; i += blockDim.x) {
+        sh[T_CS_ION + i] = kCsPIon[i];
+        sh[T_DEDX + i]   = kDedxRestricted[i];
+        sh[T_RANGE + i]  = kRangeSteps[i];
+        sh[T_CS_PP + i]  = kCsPPElastic[i];
+        sh[T_CS_POE + i] = kCsPOElastic[i];
+        sh[T_CS_POI + i] = kCsPOInelastic[i];
+    }
+    __syncthreads();
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (fn == 0) {
+        out[i] = spr<DBG>(rho[i], ek[i]);
+    } else if (fn == 1) {
+        out[i] = radiation_length<DBG>(rho[i]);
+    } else {
+        ctx c;
+        c.T           = sh;
+        const rel_q r = relq(ek[i]);
+        float*      o = out + 10 * i;
+        o[0]          = cs_ion(c, r.Ek, rho[i]);
+        o[1]          = cs_tab(c, T_CS_PP, r.Ek, rho[i]);
+        o[2]          = cs_tab(c, T_CS_POE, r.Ek, rho[i]);
+        o[3]          = cs_tab(c, T_CS_POI, r.Ek, rho[i]);
+        o[4]          = dedx(c, r.Ek);
+        o[5]          = r.beta_sq;
+        o[6]          = r.gamma;
+        o[7]          = r.Et;
+        o[8]          = r.Te_max;
+        o[9]          = momentum(r);
+    }
+}
+
+int
+physics_probe(int debug, int fn, const float* rho, const float* ek, int64_t n, float* out, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const dim3 grid((unsigned) ((n + 255) / 256));
+    if (debug)
+        hipLaunchKernelGGL(k_physics_probe<true>, grid, dim3(256), 0, stream, fn, rho, ek, n, out);
+    else
+        hipLaunchKernelGGL(k_physics_probe<false>, grid, dim3(256), 0, stream, fn, rho, ek, n, out);
+    return (int) hipGetLastError();
+}
+
 // resident workgroups per CU of one instantiation
 int
 occupancy(int debug, int dynamic, int trace, int waves, int threads, size_t shmem) {

@@ -13,6 +13,15 @@
 namespace mqi_hip
 {
 
+// The scale tps_env applies to every written scorer: particles_per_history *
+// rbe * n_fractions evaluated in float (float * float * int, the members'
+// types, mqi_tps_env.hpp:117,133-134 and :1873-1889), then widened.
+inline double
+output_scale(float particles_per_history, float rbe, int n_fractions) {
+    const float s = particles_per_history * rbe * n_fractions;
+    return (double) s;
+}
+
 // tps_env without DICOM (mqi_tps_env.hpp): the inputs of its parameter file
 // that do not need GDCM, under the same names, and its world, run and output
 // steps on a MetaImage CT with a caller-made beam source.
@@ -48,7 +57,8 @@ public:
     std::vector<beamline_grid> beamline;
     std::vector<mqi_spot>      beamlets;
     std::string                output_dir, output_format = "raw", beam_name = "beam";
-    float                      particles_per_history = 1.0f, rbe = 1.1f, n_fractions = 1.0f;
+    float                      particles_per_history = 1.0f, rbe = 1.1f;
+    int                        n_fractions           = 1;
     uint64_t                   random_seed           = 0;
     // -1: the kernel tps_env runs (:1118-1135): transport_particles_patient_stat
     // with the statistics pair, transport_particles_patient otherwise (which
@@ -119,7 +129,7 @@ public:
     finalize() override {
         x_environment::finalize();
         if (output_dir.empty()) return;
-        const double      scale  = (double) particles_per_history * rbe * n_fractions;
+        const double      scale  = output_scale(particles_per_history, rbe, n_fractions);
         const std::string prefix = beam_name + "_" + std::to_string(beamline.size()) + "_";
         if (dij_) {
             check(ctx_, mqi_dij_build(ctx_, 0.0), "mqi_dij_build");

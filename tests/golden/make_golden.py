@@ -306,8 +306,104 @@ def ct_fixture(threads, per_spot=10000, n_batches=8, let_per_spot=0, let_batches
     return out
 
 
+DOSE_Q = 16384.0   # volume fixtures: dose / max stored as round(value * DOSE_Q) in uint16
+
+
+def _quantise(d, s):
+    """Dose relative to its maximum as uint16 steps of 1 / DOSE_Q (error
+    <= 3.1e-5 of the maximum), the standard error relative to the same
+    maximum as float16 with 4 significand bits (relative error <= 3 %,
+    enough for a 2-sigma bar; the dropped bits are noise that would not
+    compress)."""
+    q = np.rint(np.clip(d, 0.0, 1.0) * DOSE_Q).astype(np.uint16)
+    sb = s.astype(np.float16).view(np.uint16)
+    sb = ((sb.astype(np.uint32) + 0x20) & 0xffc0).astype(np.uint16)   # round to 4 significand bits
+    return q, sb.view(np.float16)
+
+
+def _box_rel(d3, s3, margin):
+    """The bounding box of the voxels at or above 10 % of the maximum, grown
+    by `margin` voxels per axis (the gamma search reach), and the mean and
+    standard error inside it relative to the maximum (_quantise)."""
+    margin = np.broadcast_to(np.asarray(margin), (3,))
+    hot = np.argwhere(d3 >= 0.1 * d3.max())
+    lo = np.maximum(hot.min(0) - margin, 0)
+    hi = np.minimum(hot.max(0) + margin + 1, d3.shape)
+    sl = tuple(slice(a, b) for a, b in zip(lo, hi))
+    q, s = _quantise(d3[sl] / d3.max(), s3[sl] / d3.max())
+    return np.concatenate([lo, hi]).astype(np.int32), np.array(d3.max()), q, s
+
+
+def rebin_xy(a3, f):
+    """Sum of f x f lateral (y, x) voxel blocks of a (z, y, x) array divided
+    by f * f: the dose of the coarser voxel."""
+    nz, ny, nx = a3.shape
+    return a3.reshape(nz, ny // f, f, nx // f, f).sum((2, 4)) / (f * f)
+
+
+def volumes_fixture(threads, c2_per_spot=312500, c3_per_spot=40000, n_batches=8, only=None):
+    """Volumes for 3-D gamma tests.
+
+    c2  the slab workload (25 spots 70-230 MeV, release physics), n_batches x
+        c2_per_spot histories per spot, dose; the 0.5 x 0.5 x 1 mm voxels
+        rebinned laterally 2 x 2 into 1 x 1 x 1 mm (rebin_xy), kept in the box
+        above 10 % of the maximum plus 2 voxels.
+    c3  the 100-spot sub-plan on the cropped CT (phantoms.ct_crop_plan), n_batches x
+        c3_per_spot histories per spot, dose on the 0.98 x 0.98 x 2 mm voxels,
+        kept in the box above 10 % of the maximum plus 2 voxels (y, x) and
+        1 voxel (z).
+    Mean and batch standard error per voxel relative to the maximum
+    (_quantise; Gy per history = maximum x dose_rel / DOSE_Q)."""
+    from moquimc_amd import workloads
+    path = os.path.join(HERE, "reference_volumes.npz")
+    out = dict(np.load(path)) if (only and os.path.exists(path)) else {}
+    if only in (None, "c2"):
+        t0 = time.time()
+        w = workloads.slabs()
+        g = rb.Grid(w.xe, w.ye, w.ze, w.density)
+        runs = _batches(g, [s.copy() for s in w.spots], 0, c2_per_spot, n_batches, rb.SC_DOSE, threads, 6000)
+        vols = [rebin_xy(r[0].reshape(350, 200, 200), 2) for r in runs]
+        m, se = _mean_se(vols)
+        box, scale, d, s = _box_rel(m, se, 2)
+        out.update({"c2_box": box, "c2_dose_scale": scale, "c2_dose_rel": d, "c2_dose_se_rel": s,
+                    "c2_histories": np.array(n_batches * c2_per_spot * len(w.spots)),
+                    "c2_batches": np.array(n_batches)})
+        print("c2 volume done in %.0f s" % (time.time() - t0), flush=True)
+    if only in (None, "c3"):
+        t0 = time.time()
+        xe, ye, ze, hu, spots = phantoms.ct_crop_plan(c3_per_spot)
+        nz, ny, nx = hu.shape
+        rho = np.zeros(hu.size, np.float32)
+        h = np.ascontiguousarray(hu.ravel())
+        rb.reference(0).ref_hu_to_density(rb.ptr(h, rb.ctypes.c_void_p), rb.ctypes.c_int64(h.size),
+                                          rb.ptr(rho, rb.c_f))
+        g = rb.Grid(xe, ye, ze, rho)
+        vols = []
+        for b in range(n_batches):
+            vtx, seeds = [], []
+            for k, sp in enumerate(spots):
+                v, _ = rb.oracle_sample_spot(sp, 7000 + b, k, c3_per_spot, rng=0)
+                vtx.append(v)
+                seeds.append(np.random.default_rng([7000 + b, k]).integers(1, 2 ** 31 - 1,
+                                                                            size=c3_per_spot).astype(np.int32))
+            arrs, tracked = rb.ref_transport(0, g, np.concatenate(vtx), np.concatenate(seeds), mask=rb.SC_DOSE,
+                                             kernel=0, threads=threads)
+            vols.append(arrs[0].reshape(nz, ny, nx) / tracked)
+            print("  c3 batch %d: %d histories, %.0f s" % (b, tracked, time.time() - t0), flush=True)
+        m, se = _mean_se(vols)
+        box, scale, d, s = _box_rel(m, se, [1, 2, 2])
+        out.update({"c3_box": box, "c3_dose_scale": scale, "c3_dose_rel": d, "c3_dose_se_rel": s,
+                    "c3_histories": np.array(n_batches * c3_per_spot * len(spots)),
+                    "c3_batches": np.array(n_batches)})
+        print("c3 volume done in %.0f s" % (time.time() - t0), flush=True)
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--volumes", choices=["all", "c2", "c3"], default=None,
+                    help="write reference_volumes.npz (3-D dose of the slab workload and the CT sub-plan)")
     ap.add_argument("--stats", action="store_true")
     ap.add_argument("--only", choices=["c2"], default=None, help="with --stats: recompute the slab workload only")
     ap.add_argument("--ct", action="store_true", help="write reference_ct.npz (the cropped-CT sub-plan)")
@@ -317,6 +413,9 @@ def main():
     args = ap.parse_args()
     if not rb.have_reference():
         raise SystemExit("build the reference harness first: make -C oracle")
+    if args.volumes:
+        volumes_fixture(args.threads, only=None if args.volumes == "all" else args.volumes)
+        return
     if args.ct:
         path = os.path.join(HERE, "reference_ct.npz")
         if args.ct_let_only:

@@ -1,10 +1,12 @@
 // Test driver for the host beam model (moquimc_amd/host/mqi_hip_pbs.hpp):
 //   pbs_driver <machine file> <source_to_isocenter (hex float)> <particles_per_history (hex float)> <spots file>
+//              [beamsource.bin]
 // spots file: one "e x y meterset" line per spot, hex floats.  Prints per
 // spot either "error" or the beamlet as hex floats:
 //   energy sigma_energy mean[6] sigma[6] histories
 // and, last, the create_beamsource table of the whole list with identity
-// rotation and zero translation as "beamsource <n> <sum of histories>".
+// rotation and zero translation as "beamsource <n> <sum of histories>"; with
+// a fifth argument the table's mqi_spot records are also written there.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -15,7 +17,7 @@
 
 int
 main(int argc, char** argv) {
-    if (argc != 5) return 2;
+    if (argc != 5 && argc != 6) return 2;
     try {
         mqi_hip::pbs_model m(argv[1]);
         m.source_to_isocenter_mm = std::strtof(argv[2], nullptr);
@@ -43,6 +45,11 @@ main(int argc, char** argv) {
         unsigned long long         sum = 0;
         for (const mqi_spot& b : bs) sum += b.histories;
         std::printf("beamsource %zu %llu\n", bs.size(), sum);
+        if (argc == 6) {
+            std::ofstream o(argv[5], std::ios::binary);
+            o.write(reinterpret_cast<const char*>(bs.data()), (std::streamsize) (bs.size() * sizeof(mqi_spot)));
+            if (!o) return 1;
+        }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

@@ -181,3 +181,44 @@ def test_ct_env_driver_builds_and_fails_cleanly_without_device(tmp_path):
     r = subprocess.run([exe, "/nonexistent.mha", str(spots), str(tmp_path), "raw", "1", "Dose", "0", "-"],
                        capture_output=True, text=True)
     assert r.returncode == 1 and r.stderr.strip()
+
+
+def test_ct_env_output_scale_is_the_float_product(tmp_path):
+    """ct_env scales its outputs by particles_per_history * rbe * n_fractions
+    evaluated as tps_env does, in float with an int fraction count
+    (mqi_tps_env.hpp:117,133-134,1873-1889): equal to numpy's float32
+    product, and for RBE 1.1 over 30 fractions different from the double
+    product (so the test sees the difference)."""
+    src = tmp_path / "scale.cpp"
+    src.write_text('#include <cstdio>\n#include "mqi_hip_ct_env.hpp"\n'
+                   'int main(){ const float p[] = {1.0f, 1.0f, 0.5f, 1e8f, 3.7e4f};\n'
+                   ' const float r[] = {1.1f, 1.1f, 1.0f, 1.1f, 1.05f}; const int n[] = {1, 30, 7, 33, 19};\n'
+                   ' for (int i = 0; i < 5; ++i) std::printf("%.17g\\n", mqi_hip::output_scale(p[i], r[i], n[i]));\n'
+                   ' return 0; }\n')
+    exe = str(tmp_path / "scale")
+    lib = os.path.join(ROOT, "moquimc_amd")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-I", HOST,
+                    str(src), "-L", lib, "-lmqi_hip", "-Wl,-rpath," + lib, "-o", exe], check=True)
+    got = [float(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    p = np.float32([1.0, 1.0, 0.5, 1e8, 3.7e4])
+    r = np.float32([1.1, 1.1, 1.0, 1.1, 1.05])
+    n = np.float32([1, 30, 7, 33, 19])
+    want = (p * r * n).astype(np.float64)
+    assert got == list(want)
+    assert got[1] == 33.0 and float(np.float32(1.1)) * 30 != 33.0   # the double product of the same inputs
+
+
+def test_dij_size_guard_on_host(tmp_path):
+    """mqi_dij_build refuses a scenario's Dij of more than 2^31 - 1 nonzeros
+    with MQI_E_CAPACITY (its row sort counts items in 32-bit ints) before it
+    allocates or sorts anything: the guard at the boundary, on the host."""
+    src = tmp_path / "guard.cpp"
+    src.write_text('#include <cstdio>\n#include "mqi_limits.h"\n'
+                   'int main(){ const long long n[] = {0, 1, 2147483646ll, 2147483647ll, 2147483648ll, '
+                   '4000000000ll, -1};\n for (long long v : n) std::printf("%d\\n", dij_size_check(v));\n'
+                   ' return 0; }\n')
+    exe = str(tmp_path / "guard")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "moquimc_amd", "csrc"),
+                    str(src), "-o", exe], check=True)
+    got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [0, 0, 0, 0, 5, 5, 5]   # MQI_OK ... MQI_E_CAPACITY

@@ -210,3 +210,30 @@ def test_save_npz_and_dense_formats(ctx, tmp_path):
     assert np.array_equal(np.frombuffer(blob[start:], np.float64), dose * 2.0)
     with pytest.raises(Exception):
         ctx.save_scorer(0, str(tmp_path / "no_such_dir"), "x", fmt="raw")
+
+
+def test_forward_product_is_reproducible_across_builds(ctx):
+    """D w runs on a voxel-major copy whose voxel rows list their spots in
+    ascending order (a stable sort, not atomic cursors), so two
+    mqi_dij_build calls on the same tables give bit-identical D w, and
+    repeated products give the same bits."""
+    import torch
+    g, sp = _setup(ctx)
+    sp = np.concatenate([sp, sp])   # more spots per voxel row
+    sp["histories"] = 400
+    ctx.set_spots(sp)
+    ctx.run(ctx.config(seed=4, batched=True, dij=True, dij_capacity=1 << 15))
+    dev = torch.device("cuda", 0)
+    w = torch.from_numpy(np.random.default_rng(1).uniform(0.5, 1.5, len(sp)).astype(np.float32)).to(dev)
+    outs = []
+    for _ in range(2):
+        ctx.dij_build(0.0)
+        for _ in range(2):
+            d = torch.zeros(g.nvox, dtype=torch.float64, device=dev)
+            ctx.dij_matvec(0, 0, w.data_ptr(), d.data_ptr(), y_double=True)
+            torch.cuda.synchronize()
+            outs.append(d.cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
+    D = _csr(ctx)
+    assert np.allclose(outs[0], D.T @ w.cpu().numpy().astype(np.float64), rtol=1e-9, atol=0)

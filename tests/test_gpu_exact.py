@@ -375,3 +375,40 @@ def test_exact_robust_scenarios_match_oracle_on_shifted_and_scaled_grids(ctx):
     assert not np.allclose(ctx.scorer(0, 1), ctx.scorer(0, 0))
     assert not np.allclose(ctx.scorer(0, 2), ctx.scorer(0, 0))
     ctx.set_scenarios()
+
+
+@pytest.mark.parametrize("loops,kernel,mask", [
+    (0, 0, rb.SC_DOSE | rb.SC_LETD),                  # transport_particles_patient, 3 scorers
+    (0, 0, rb.SC_DOSE | rb.SC_LETD | rb.SC_STAT),     # the same with the statistics pair (5 scorers)
+    (1, 1, rb.SC_DOSE | rb.SC_LETD | rb.SC_STAT),     # transport_particles_patient_stat
+])
+def test_exact_reference_scoring_loops_match_oracle(ctx, loops, kernel, mask):
+    """The reference's two scoring loops (MQI_LOOPS_REFERENCE_PLAIN: scorers
+    0..n-3 scored in both of the plain kernel's loops when n >= 3,
+    mqi_transport.hpp:204-225; MQI_LOOPS_REFERENCE_STAT: the last two
+    through get_mask_idx, :343-364) on the device equal the oracle's plain
+    and _stat kernels; and the plain loop's scorers 0..n-3 are twice the
+    once-per-step values (the double count itself)."""
+    g = phantoms.slab_box(30, 30, 200)
+    sp = phantoms.spots([120.0, 160.0], 300, sigma=4.0, sigma_e_frac=0.01, positions=[(0, 0, 0.5), (5, -3, 0.5)])
+    ref, ntr = rb.oracle_run_spots(g, sp, mask=mask, kernel=kernel, prec=32, rng=1, math=1, debug=0, master_seed=44,
+                                   threads=4)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(sp)
+    ctx.set_scorers(mask)
+    n_sc = ctx.scorer_count()
+    assert n_sc == len(ref)
+    res = ctx.run(ctx.config(seed=44, exact=True, batched=True, loops=loops))
+    assert res.histories == ntr == 600
+    got = [ctx.scorer(k) for k in range(n_sc)]
+    for k in range(n_sc):
+        fin = np.isfinite(ref[k])
+        assert np.array_equal(np.isfinite(got[k]), fin), k
+        assert np.allclose(got[k][fin], ref[k][fin], rtol=1e-9, atol=0), k
+    if loops == 0:
+        ctx.zero_scorers()
+        ctx.run(ctx.config(seed=44, exact=True, batched=True, loops=2))
+        for k in range(n_sc - 2):
+            once = ctx.scorer(k)
+            once[0] = 0.0   # voxel 0 is never scored by the reference's scorers
+            assert np.allclose(got[k], 2.0 * once, rtol=1e-9, atol=0), k

@@ -76,3 +76,64 @@ def test_full_ct_plan_dose_is_additive_over_spots(ctx):
     assert res.status == 0 and res.histories == w.histories
     plain = ctx.scorer(0)
     assert np.allclose(plain, full, rtol=1e-9, atol=1e-12 * full.max())
+
+
+def test_full_robust_plan_nine_scenarios():
+    """The 2000-spot CT plan under 9 robust scenarios in one launch (nominal,
+    +-3 mm shifts in x, y, z, density x 1.035 and 0.965; the nominal scenario
+    with twice the histories; the XShift/YShift/ZShift and DensityScaling
+    scenarios of tps_env, mqi_tps_env.hpp:475-477,766-769), Dij scorer with a
+    per-scenario dose beside it:
+    - every history of every scenario is transported (sum of budget x spot
+      histories);
+    - each scenario's Dij (threshold 0) sums over its spot rows to that
+      scenario's dose (float32 entries of FP64 sums);
+    - the worst-case quadratic objective's gradient through the autograd
+      layer (moquimc_amd.dij) equals scipy's FP64 gradient of the worst
+      scenario to 1e-5 relative."""
+    import torch
+    from moquimc_amd import _abi, dij, workloads
+    w = workloads.robust()
+    shifts, scales, budgets = w.scenarios
+    c = _abi.Context(0)
+    try:
+        w.load(c, per_scenario=True)
+        print("robust plan: %d spots x %d scenarios" % (len(w.spots), len(budgets)), flush=True)
+        res = c.run(c.config(seed=13, batched=True, dij=True, dij_slots_sqrt=4096))
+        want_h = sum(int(np.rint(float(b) * h)) for b in budgets for h in w.spots["histories"].astype(np.float64))
+        assert res.status == 0 and res.histories == want_h
+        print("transported %d histories in %.1f s" % (res.histories, res.kernel_ms / 1e3), flush=True)
+        c.dij_build(0.0)
+        n_vox = int(np.prod(w.shape))
+        for s in range(len(budgets)):
+            indptr, idx, val, shape = c.dij_csr(s)
+            assert shape == (len(w.spots), n_vox)
+            col = np.bincount(idx, weights=val.astype(np.float64), minlength=n_vox)
+            dose = c.scorer(0, s)
+            assert np.any(dose != 0)
+            assert np.allclose(col, dose, rtol=2e-6, atol=1e-6 * dose.max()), s
+            del indptr, idx, val, col, dose
+            print("scenario %d: rows sum to its dose" % s, flush=True)
+        mats = [dij.DijMatrix(c, s, scale=1.0 / float(budgets[s])) for s in range(len(budgets))]
+        dev = torch.device("cuda", 0)
+        target = torch.from_numpy(workloads.target_prescription(w.xe, w.ye, w.ze, radius=30.0, dose=1.0)).to(dev)
+        gen = torch.Generator(device="cpu").manual_seed(5)
+        wt = (torch.rand(len(w.spots), generator=gen) + 0.5).to(dev)
+        with torch.no_grad():
+            presc = target.double() * mats[0].matvec(wt)[target > 0].mean()
+        wv = wt.clone().requires_grad_(True)
+        f, vals = dij.worst_case_objective(mats, wv, presc)
+        f.backward()
+        k = int(torch.argmax(vals).item())
+        indptr, idx, val, shape = c.dij_csr(k)
+        D = sps.csr_matrix((val.astype(np.float64) / float(budgets[k]), idx, indptr), shape=shape)
+        wn = wt.double().cpu().numpy()
+        r = D.T @ wn - presc.cpu().numpy()
+        assert abs(float(f) - float(r @ r) / n_vox) <= 1e-6 * abs(float(r @ r) / n_vox)
+        g_ref = 2.0 * (D @ r) / n_vox
+        g = wv.grad.double().cpu().numpy()
+        err = float(np.abs(g - g_ref).max() / np.abs(g_ref).max())
+        print("worst scenario %d, gradient vs scipy FP64: %.3g relative" % (k, err), flush=True)
+        assert err <= 1e-5
+    finally:
+        c.close()

@@ -288,3 +288,55 @@ def test_ct_subplan_letd_against_reference(ctx):
         keep.sum(), within.mean(), np.median(rel), rel.max()))
     assert within.mean() >= 0.9
     assert np.median(rel) <= 0.01
+
+
+def _has_fp64_batches():
+    return os.path.exists(FP64) and "idd64_batches" in np.load(FP64).files
+
+
+@pytest.mark.skipif(not _has_fp64_batches(), reason="reference_fp64_idd.npz has no FP64 batches")
+def test_production_fp32_depth_dose_against_fp64(ctx):
+    """FP32 stepping error of the production kernels (hardware division,
+    square root, log2 / exp2 pow, FMA contraction, float relativistic
+    quantities) on the 200 MeV water depth dose, against the oracle run in
+    double (tests/golden/make_fp64_idd.py --batches: 8 x 1e5 histories).
+    The two use different arithmetic, so their histories part ways and the
+    comparison is statistical: 8 x 1e6 production histories; R80 within
+    0.1 mm and within 3 combined batch errors, the depth dose within 3
+    combined errors at >= 90 % of the depths above 10 % of its maximum, the
+    integral dose within 0.5 % and within 3 combined errors."""
+    from moquimc_amd import workloads
+    ref = np.load(FP64)
+    b64 = ref["idd64_batches"]
+    w = workloads.water(histories=1000000)
+    ctx.set_grid(w.xe, w.ye, w.ze, w.density)
+    ctx.set_spots(w.spots)
+    ctx.set_scorers(1)
+    b32 = []
+    for b in range(8):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=78, batched=True, exact=False, physics_debug=True, hist_offset=b * 1000000))
+        assert res.histories == 1000000 and res.status == 0
+        b32.append(ctx.scorer(0).reshape(350, -1).sum(1) / res.histories)
+    b32 = np.array(b32)
+    depth = _depth(w.ze)
+    m32, s32 = _mean_se(b32)
+    m64, s64 = _mean_se(b64)
+    r80_32 = np.array([dosecmp.r80(depth, x[::-1]) for x in b32])
+    r80_64 = np.array([dosecmp.r80(depth, x[::-1]) for x in b64])
+    d_r80 = dosecmp.r80(depth, m32[::-1]) - dosecmp.r80(depth, m64[::-1])
+    s_r80 = np.sqrt(r80_32.var(ddof=1) / len(r80_32) + r80_64.var(ddof=1) / len(r80_64))
+    keep = m64 >= 0.1 * m64.max()
+    s = np.sqrt(s32 ** 2 + s64 ** 2)
+    ok = np.abs(m32 - m64) <= 3.0 * s
+    k = int(np.argmax(np.abs(m32 - m64) * keep))
+    t32, t64 = b32.sum(1), b64.sum(1)
+    ratio = t32.mean() / t64.mean()
+    s_ratio = ratio * np.sqrt(t32.var(ddof=1) / len(t32) / t32.mean() ** 2 + t64.var(ddof=1) / len(t64) / t64.mean() ** 2)
+    print("production FP32 - FP64: R80 %+.4f +- %.4f mm; max |dIDD| %.2e of max (%.1f sigma, depth %.1f mm); "
+          "within 3 sigma at %.3f of %d depths; integral ratio %.5f +- %.5f" % (
+              d_r80, s_r80, abs(m32[k] - m64[k]) / m64.max(), abs(m32[k] - m64[k]) / s[k], depth[::-1][k],
+              ok[keep].mean(), keep.sum(), ratio, s_ratio))
+    assert abs(d_r80) <= 0.1 and abs(d_r80) <= 3.0 * s_r80 + 0.02
+    assert ok[keep].mean() >= 0.9
+    assert abs(ratio - 1.0) <= 0.005 and abs(ratio - 1.0) <= 3.0 * s_ratio + 1e-4
