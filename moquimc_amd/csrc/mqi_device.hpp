@@ -166,7 +166,13 @@ MQI_DI rel_q relq(float Ek) {
     return r;
 }
 MQI_DI float momentum(const rel_q& r) {
+#if MQI_EXACT
     return MQ_SQRTF(r.Et * r.Et - units::Mp * units::Mp);
+#else
+    // production: p^2 = Ek (Ek + 2 Mp), the same quantity without the
+    // cancellation of Et^2 - Mp^2 in float at low energy
+    return MQ_SQRTF(r.Ek * (r.Ek + 2.0f * units::Mp));
+#endif
 }
 
 MQI_DI float intpl(float x, float x0, float x1, float y0, float y1) {

@@ -142,6 +142,38 @@ def test_exact_dose_letd_matches_oracle(ctx, debug, batched):
         assert np.allclose(got[fin], want[fin], rtol=1e-9, atol=0), k
 
 
+@pytest.mark.parametrize("batched", [True, False])
+def test_exact_run_vertices_matches_oracle(ctx, batched):
+    """Caller-made vertices with per-vertex spot ids (upload_vertices +
+    scorer_offset_vector, mqi_upload_data.hpp:415-420, mqi_transport.hpp:150-154):
+    every history keyed by (seed, vertex index, spot id) in both; doses and
+    LETd equal the oracle's up to FP64 summation order."""
+    g = phantoms.slab_box(30, 30, 200)
+    rng = np.random.default_rng(17)
+    n = 1200
+    v = np.zeros((n, 7), np.float32)
+    v[:, 0] = rng.uniform(70.0, 170.0, n)
+    v[:, 1:3] = rng.normal(0.0, 3.0, (n, 2))
+    v[:, 3] = 0.5
+    d = np.c_[rng.normal(0.0, 0.02, (n, 2)), -np.ones(n)]
+    v[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    sid = rng.integers(0, 3, n).astype(np.uint32)
+    mask = rb.SC_DOSE | rb.SC_LETD
+    ref, ntr = rb.oracle_transport(g, v, np.zeros(n, np.int32), spot_ids=sid, mask=mask, kernel=2, threads=4,
+                                   prec=32, rng=1, math=1, debug=0, master_seed=23)
+    ctx.set_grid(g.xe, g.ye, g.ze, g.density)
+    ctx.set_spots(phantoms.spots([100.0, 110.0, 120.0], 1))
+    ctx.set_scorers(mask)
+    ctx.zero_scorers()
+    res = ctx.run_vertices(v, ctx.config(seed=23, exact=True, batched=batched), spot_ids=sid)
+    assert res.histories == ntr == n
+    for k in range(3):
+        got, want = ctx.scorer(k), ref[k]
+        fin = np.isfinite(want)
+        assert np.array_equal(np.isfinite(got), fin)
+        assert np.allclose(got[fin], want[fin], rtol=1e-9, atol=0), k
+
+
 def test_fresh_context_first_run_binds_work_list():
     """Regression: the work-list arrays are (re)allocated by the run itself;
     the launch must bind them after that, also on a context's first run."""

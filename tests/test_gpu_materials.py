@@ -146,9 +146,10 @@ def test_cross_sections_dedx_relativistic_on_device(ctx, rho):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     # production: cross sections and dEdx are table interpolations with the
     # hardware divide (relative 1e-6); Et and gamma are the same float
-    # expressions (2 ulp); beta^2, Te_max and momentum use beta^2 = Ek (Ek +
-    # 2 Mp) / Et^2, which avoids the reference's cancellation in 1 - 1 /
-    # gamma^2: compared with the closed form in double (relative 1e-5)
+    # expressions (2 ulp); beta^2 and Te_max use beta^2 = Ek (Ek + 2 Mp) /
+    # Et^2 and the momentum sqrt(Ek (Ek + 2 Mp)), which avoid the reference's
+    # cancellations in 1 - 1 / gamma^2 and Et^2 - Mp^2 at low energy:
+    # compared with the closed form in double (relative 1e-5)
     got = ctx.physics_probe(2, r, e, exact=False)
     t = want.astype(np.float64)
     rel = np.abs(got[:, :5] - t[:, :5]) / np.maximum(np.abs(t[:, :5]), 1e-30)
