@@ -293,6 +293,24 @@ int mqi_math_probe(mqi_ctx* ctx, int fn, const double* x, const double* y, int64
 int mqi_physics_probe(mqi_ctx* ctx, int exact, int physics_debug, int fn, const float* rho, const float* ek, int64_t n,
                       float* out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mqi_run_vertices with the vertices (7 floats each) and
+ * the optional spot ids already in DEVICE memory, and nothing allocated.  The
+ * one-item work list, the counters, the work-range queue and the per-lane
+ * secondary stacks live in a static per-GPU scratch array of
+ * MQI_NOALLOC_STACK_BYTES (+ 8 KiB), held from library load to process end;
+ * the launch takes as many lanes as that holds stacks for (stack_depth x 40 B
+ * per lane), at most the launch mqi_run_vertices makes.  A stack_depth x
+ * block_size above the scratch is refused with MQI_E_CAPACITY.  Scores into
+ * the scorers set with mqi_set_scorers, as mqi_run_vertices does; results
+ * equal mqi_run_vertices' to FP64 summation order.  Every call on one GPU
+ * shares that GPU's array (counters are cleared at the start of each call):
+ * calls that overlap on one GPU race. */
+#define MQI_NOALLOC_STACK_BYTES (32u << 20)
+int mqi_run_vertices_noalloc(mqi_ctx* ctx, const float* d_vertices, uint64_t n, const uint32_t* d_spot_ids,
+                             const mqi_run_config* cfg, mqi_run_result* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,7 @@ def lib():
             "mqi_scorer_count": (i, [vp]),
             "mqi_run": (i, [vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
             "mqi_run_vertices": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
+            "mqi_run_vertices_noalloc": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
             "mqi_trace": (i, [vp, ctypes.POINTER(RunConfig), vp, u64, i, vp, vp, vp]),
             "mqi_get_scorer": (i, [vp, i, i, vp]),
             "mqi_scorer_device_ptr": (i, [vp, i, i, ctypes.POINTER(vp)]),
@@ -108,7 +109,7 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
             "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha",
-            "mqi_physics_probe"]
+            "mqi_physics_probe", "mqi_run_vertices_noalloc"]
 
 
 class GridDesc(ctypes.Structure):
@@ -302,6 +303,16 @@ class Context:
         res = RunResult()
         self._chk(self._L.mqi_run_vertices(self.h, _p(v), len(v), _p(sid), ctypes.byref(cfg), ctypes.byref(res)),
                   "mqi_run_vertices")
+        return res
+
+    def run_vertices_noalloc(self, d_vertices, n, cfg, d_spot_ids=None):
+        """mqi_run_vertices_noalloc: d_vertices (n x 7 float32) and d_spot_ids
+        (n uint32, or None) are device addresses (e.g. a torch tensor's
+        data_ptr()); nothing is allocated by the library."""
+        res = RunResult()
+        self._chk(self._L.mqi_run_vertices_noalloc(self.h, ctypes.c_void_p(d_vertices), n,
+                                                   None if d_spot_ids is None else ctypes.c_void_p(d_spot_ids),
+                                                   ctypes.byref(cfg), ctypes.byref(res)), "mqi_run_vertices_noalloc")
         return res
 
     def trace(self, cfg, n_histories, max_steps, vertices=None):

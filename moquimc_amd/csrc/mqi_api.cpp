@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -133,7 +134,58 @@ struct mqi_ctx {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
     bool       matvec_pending = false;
+
+    // this device's copy of the no-allocation scratch array (see scratch below)
+    unsigned char* scratch     = nullptr;
+    std::string    scratch_err;
 };
+
+// ------------------------------------------------------------------ no-allocation scratch
+// mqi_run_vertices_noalloc keeps its one-item work list, counters, work-range
+// queue and per-lane secondary stacks in this array instead of buffers the
+// library allocates.  Each GPU holds its own copy from module load to process
+// end.  In host code the name stands for no device address: the per-device
+// address comes from hipGetSymbolAddress (scratch_address) and only that
+// address is passed to kernels and fills.
+#define MQI_SCRATCH_HEADER 8192
+static __device__ __align__(256) unsigned char scratch[MQI_SCRATCH_HEADER + MQI_NOALLOC_STACK_BYTES];
+
+namespace
+{
+
+// byte offsets inside the header
+constexpr size_t SCR_COUNTERS = 0;      // 4 x u64: -, tracked, dropped, steps
+constexpr size_t SCR_STATUS   = 256;    // u32
+constexpr size_t SCR_HOFF     = 512;    // u64
+constexpr size_t SCR_ITEM_END = 768;    // u64
+constexpr size_t SCR_ITEM_SPT = 1024;   // u32
+constexpr size_t SCR_ITEM_SCN = 1280;   // u32
+constexpr size_t SCR_QUEUE    = 4096;   // MQI_QUEUES * MQI_QUEUE_STRIDE u64
+static_assert(SCR_QUEUE + sizeof(unsigned long long) * MQI_QUEUES * MQI_QUEUE_STRIDE <= MQI_SCRATCH_HEADER,
+              "the work-range queue must fit the scratch header");
+
+constexpr int MQI_SCRATCH_DEVICES = 64;
+std::mutex    g_scratch_mu;
+void*         g_scratch_addr[MQI_SCRATCH_DEVICES] = { nullptr };
+
+// The address of `scratch` on `device` (which must be current), looked up
+// once per device and kept.
+hipError_t
+scratch_address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= MQI_SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void*      p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+
+}   // namespace
 
 namespace
 {
@@ -421,10 +473,11 @@ bind_buffers(mqi_ctx* c, transport_params& P) {
     P.stack         = c->d_stack;
 }
 
+// A launch over the buffers P already names (work list, queue, counters,
+// stacks); stack_bytes is the size of the array P.stack points into.
 int
-launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
-       int threads) {
-    bind_buffers(c, P);
+launch_bound(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
+             int threads, size_t stack_bytes) {
     if (P.fetch_chunk < 1) return (int) hipErrorInvalidValue;
     if (!P.density || !P.xe || !P.ye || !P.ze || !P.stack || !P.queue || !P.tracked || !P.drops || !P.steps || !P.status ||
         !P.hist_offset || (!P.vertices && (!P.spots || !P.item_end || !P.item_spot || !P.item_scenario)))
@@ -432,17 +485,24 @@ launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, 
     for (int s = 0; s < P.n_scorers; ++s)
         if (P.kinds[s] == MQI_KIND_DIJ ? (!P.dij_keys || !P.dij_vals) : !P.out[s]) return (int) hipErrorInvalidValue;
     if (trace && (!P.trace_cnb || !P.trace_rec || !P.trace_n)) return (int) hipErrorInvalidValue;
-    if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float) > c->stack_bytes)
+    if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float) > stack_bytes)
         return (int) hipErrorInvalidValue;
     const size_t sh = shmem_bytes(c);
     if (dynamic) {
-        const int e = mqi_aux::queue_init(c->d_queue, P.total, c->stream);
+        const int e = mqi_aux::queue_init(P.queue, P.total, c->stream);
         if (e) return e;
     }
     if (cfg.exact)
         return mqi_exact::launch_transport(P, cfg.physics_debug, dynamic, trace, 0, blocks, threads, sh, c->stream);
     return mqi_fast::launch_transport(P, cfg.physics_debug, dynamic, trace, cfg.waves_per_simd, blocks, threads, sh,
                                       c->stream);
+}
+
+int
+launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
+       int threads) {
+    bind_buffers(c, P);
+    return launch_bound(c, P, cfg, dynamic, trace, blocks, threads, c->stack_bytes);
 }
 
 int
@@ -737,6 +797,13 @@ mqi_create(int device, mqi_ctx** out) {
     }
     hipMemcpy(c->d_shift, zero3, sizeof zero3, hipMemcpyHostToDevice);
     hipMemcpy(c->d_scale, &one, sizeof one, hipMemcpyHostToDevice);
+    // this device's no-allocation scratch (mqi_run_vertices_noalloc); a
+    // failed lookup only disables that entry point
+    void* sp = nullptr;
+    if ((e = scratch_address(device, &sp)) == hipSuccess)
+        c->scratch = (unsigned char*) sp;
+    else
+        c->scratch_err = hipGetErrorString(e);
     *out = c;
     return 0;
 }
@@ -1169,6 +1236,81 @@ mqi_run_vertices(mqi_ctx* c, const float* vtx, uint64_t n, const uint32_t* spot_
     HIPCHK(c, hipMemcpy(cnt, c->d_counters, sizeof cnt, hipMemcpyDeviceToHost));
     hipFree(d_v);
     if (d_s) hipFree(d_s);
+    if (res) {
+        std::memset(res, 0, sizeof *res);
+        res->histories = cnt[1];
+        res->dropped   = cnt[2];
+        res->steps     = cnt[3];
+        res->rounds    = 1;
+        res->kernel_ms = t;
+        res->stat      = -1.0;
+    }
+    return 0;
+}
+
+int
+mqi_run_vertices_noalloc(mqi_ctx* c, const float* d_vtx, uint64_t n, const uint32_t* d_spot_ids,
+                         const mqi_run_config* cfg_in, mqi_run_result* res) {
+    int r = ready(c);
+    if (r) return r;
+    if (!d_vtx || !cfg_in || n == 0) return fail(c, MQI_E_ARG, "bad vertex arguments");
+    if (n >= (1ull << 32)) return fail(c, MQI_E_ARG, "more than 2^32 vertices");
+    if (c->kinds.empty()) return fail(c, MQI_E_STATE, "no scorers");
+    if (!c->scratch)
+        return fail(c, MQI_E_HIP, "no scratch address on device %d: %s", c->device, c->scratch_err.c_str());
+    hipSetDevice(c->device);
+    mqi_run_config cfg = *cfg_in;
+    if ((r = check_tuning(c, cfg))) return r;
+    if (cfg.dij) return fail(c, MQI_E_ARG, "Dij scoring needs a spot table (mqi_run)");
+    unsigned char*      base     = c->scratch;
+    unsigned long long* counters = (unsigned long long*) (base + SCR_COUNTERS);
+    transport_params    P;
+    fill_params(c, P, cfg);
+    P.vertices      = d_vtx;
+    P.vertex_spot   = d_spot_ids;
+    P.n_items       = 1;
+    P.total         = n;
+    P.item_end      = (uint64_t*) (base + SCR_ITEM_END);
+    P.item_spot     = (uint32_t*) (base + SCR_ITEM_SPT);
+    P.item_scenario = (uint32_t*) (base + SCR_ITEM_SCN);
+    P.queue         = (unsigned long long*) (base + SCR_QUEUE);
+    P.tracked       = counters + 1;
+    P.drops         = counters + 2;
+    P.steps         = counters + 3;
+    P.status        = (unsigned int*) (base + SCR_STATUS);
+    P.hist_offset   = (uint64_t*) (base + SCR_HOFF);
+    P.stack         = (float*) (base + MQI_SCRATCH_HEADER);
+    // as many lanes as the scratch holds stacks for, at most the launch
+    // mqi_run_vertices makes
+    const int      threads    = cfg.block_size > 0 ? cfg.block_size : 256;
+    const uint64_t block_size = (uint64_t) threads * P.stack_cap * MQI_STACK_FLOATS * sizeof(float);
+    const uint64_t max_blocks = (uint64_t) MQI_NOALLOC_STACK_BYTES / block_size;
+    if (max_blocks == 0)
+        return fail(c, MQI_E_CAPACITY,
+                    "stack_depth %d x block_size %d needs %llu bytes of secondary stacks per block; the "
+                    "no-allocation scratch holds %llu", P.stack_cap, threads, (unsigned long long) block_size,
+                    (unsigned long long) MQI_NOALLOC_STACK_BYTES);
+    uint64_t blocks;
+    if (cfg.batched)
+        blocks = (uint64_t) persistent_blocks(c, cfg, threads, 0);
+    else
+        blocks = (std::min<uint64_t>(n, 1ull << 18) + threads - 1) / threads;
+    blocks = std::min(blocks, max_blocks);
+    // the work list (one item of n vertices) and the counters, by fill kernels
+    KCHK(c, mqi_aux::fill_u64(base + SCR_ITEM_END, (unsigned long long) n, 1, c->stream));
+    KCHK(c, mqi_aux::fill_u32(base + SCR_ITEM_SPT, 0u, 1, c->stream));
+    KCHK(c, mqi_aux::fill_u32(base + SCR_ITEM_SCN, 0u, 1, c->stream));
+    KCHK(c, mqi_aux::fill_u64(counters, 0ull, 4, c->stream));
+    KCHK(c, mqi_aux::fill_u32(base + SCR_STATUS, 0u, 1, c->stream));
+    KCHK(c, mqi_aux::fill_u64(base + SCR_HOFF, cfg.hist_offset, 1, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    KCHK(c, launch_bound(c, P, cfg, cfg.batched ? 1 : 0, 0, (int) blocks, threads, MQI_NOALLOC_STACK_BYTES));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+    unsigned long long cnt[4] = { 0, 0, 0, 0 };
+    HIPCHK(c, hipMemcpy(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost));
     if (res) {
         std::memset(res, 0, sizeof *res);
         res->histories = cnt[1];
