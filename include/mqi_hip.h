@@ -293,6 +293,13 @@ int mqi_math_probe(mqi_ctx* ctx, int fn, const double* x, const double* y, int64
 int mqi_physics_probe(mqi_ctx* ctx, int exact, int physics_debug, int fn, const float* rho, const float* ek, int64_t n,
                       float* out);
 
+/* Measurement probe (no counterpart in the reference): the time of n_updates
+ * FP64 atomic adds to uniformly hashed slots of an n_slots array
+ * (1 <= n_slots <= 2^32), the transport's scoring stream without the rest of
+ * the step; *ms is the device time of one pass after a warm-up pass.  The
+ * throughput ceiling the transport's scoring atomics are measured against. */
+int mqi_atomic_probe(mqi_ctx* ctx, int64_t n_slots, int64_t n_updates, double* ms);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): mqi_run_vertices with the vertices (7 floats each) and

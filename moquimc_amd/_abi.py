@@ -74,6 +74,7 @@ def lib():
             "mqi_run": (i, [vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
             "mqi_run_vertices": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
             "mqi_run_vertices_noalloc": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
+            "mqi_atomic_probe": (i, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(d)]),
             "mqi_trace": (i, [vp, ctypes.POINTER(RunConfig), vp, u64, i, vp, vp, vp]),
             "mqi_get_scorer": (i, [vp, i, i, vp]),
             "mqi_scorer_device_ptr": (i, [vp, i, i, ctypes.POINTER(vp)]),
@@ -109,7 +110,7 @@ EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "m
             "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
             "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha",
-            "mqi_physics_probe", "mqi_run_vertices_noalloc"]
+            "mqi_physics_probe", "mqi_run_vertices_noalloc", "mqi_atomic_probe"]
 
 
 class GridDesc(ctypes.Structure):
@@ -386,6 +387,13 @@ class Context:
         out = np.zeros_like(x)
         self._chk(self._L.mqi_math_probe(self.h, fn, _p(x), _p(y), len(x), _p(out)), "mqi_math_probe")
         return out
+
+    def atomic_probe(self, n_slots, n_updates):
+        """Device ms of n_updates FP64 atomic adds to hashed slots of an
+        n_slots array (mqi_atomic_probe)."""
+        ms = ctypes.c_double()
+        self._chk(self._L.mqi_atomic_probe(self.h, n_slots, n_updates, ctypes.byref(ms)), "mqi_atomic_probe")
+        return ms.value
 
     def physics_probe(self, fn, rho, ek=None, exact=True, physics_debug=False):
         """fn 0: spr(rho, ek); 1: radiation length(rho); 2: (n, 10) cross

@@ -21,6 +21,7 @@
 #include "mqi_aux.h"
 #include "mqi_kernels.h"
 #include "mqi_limits.h"
+#include "mqi_plan.h"
 #include "../host/mqi_hip_io.hpp"
 
 namespace
@@ -50,6 +51,19 @@ struct csr_dev {
     int32_t* nzvox = nullptr;
     int32_t* ccol  = nullptr;
     float*   xc    = nullptr;
+    // D^T g over 16-bit deltas of the compacted columns, for matrices whose
+    // rows are long (mqi_aux::pieces_staged); ccol is dropped once they are
+    // built
+    int64_t   n_piece   = 0;
+    int32_t*  pbase     = nullptr;   // n_piece: the column the piece's first delta adds to
+    int32_t*  row_piece = nullptr;   // rows + 1: each row's first piece
+    double*   ppart     = nullptr;   // n_piece: sums of the pieces
+    int64_t*  pb        = nullptr;   // n_piece + 1: staged start of each piece (whole 256-nonzero batches)
+    float*    sval      = nullptr;   // staged values (lane-major batches, zero padding)
+    uint16_t* sdcol     = nullptr;   // staged deltas
+    int32_t*  porder    = nullptr;   // n_piece: the order the product processes the pieces in
+    // D w over 16-bit spot columns (at most 65536 spots; tcol is then dropped)
+    uint16_t* tcol16 = nullptr;
 };
 
 }   // namespace
@@ -248,6 +262,14 @@ free_dij(mqi_ctx* c) {
         dfree(m.nzvox);
         dfree(m.ccol);
         dfree(m.xc);
+        dfree(m.pb);
+        dfree(m.sval);
+        dfree(m.sdcol);
+        dfree(m.porder);
+        dfree(m.pbase);
+        dfree(m.row_piece);
+        dfree(m.ppart);
+        dfree(m.tcol16);
         for (csr_plan* p : {&m.plan, &m.tplan}) {
             dfree(p->seg_k);
             dfree(p->seg_info);
@@ -303,6 +325,73 @@ build_plan(mqi_ctx* c, const std::vector<int64_t>& ip, csr_plan& P) {
     return 0;
 }
 
+// The pieces of D^T g over 16-bit column deltas (mqi_aux::pieces_staged):
+// the deltas of the compacted columns, and the positions where a row skips
+// more than 65535 columns, come from the device; each row is cut there and
+// every MQI_PIECE nonzeros (cut_pieces), then the pieces are restaged in
+// whole lane-major batches and ordered by their first columns.
+int
+build_pieces(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& ip) {
+    const int64_t       rows   = m.rows;
+    unsigned long long* d_ngap = nullptr;
+    int64_t *           d_gaps = nullptr, *d_pk = nullptr;
+    uint16_t*           d_dcol = nullptr;
+    int                 r      = 0;
+    struct guard {
+        unsigned long long*& n;
+        int64_t *&           g, *&k;
+        uint16_t*&           d;
+        ~guard() {
+            hipFree(n);
+            hipFree(g);
+            hipFree(k);
+            hipFree(d);
+        }
+    } g{ d_ngap, d_gaps, d_pk, d_dcol };
+    int64_t cap = 1 << 16;
+    if ((r = dalloc(c, &d_dcol, m.nnz)) || (r = dalloc(c, &d_ngap, 1)) || (r = dalloc(c, &d_gaps, cap))) return r;
+    unsigned long long n_gaps = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        KCHK(c, mqi_aux::deltas16(m.indptr, m.ccol, rows, m.nnz, d_dcol, d_ngap, d_gaps, cap, c->stream));
+        HIPCHK(c, hipMemcpy(&n_gaps, d_ngap, sizeof n_gaps, hipMemcpyDeviceToHost));
+        if ((int64_t) n_gaps <= cap) break;
+        hipFree(d_gaps);
+        d_gaps = nullptr;
+        cap    = (int64_t) n_gaps;
+        if ((r = dalloc(c, &d_gaps, cap))) return r;
+    }
+    std::vector<int64_t> gaps(n_gaps);
+    if (n_gaps) HIPCHK(c, hipMemcpy(gaps.data(), d_gaps, sizeof(int64_t) * n_gaps, hipMemcpyDeviceToHost));
+    std::sort(gaps.begin(), gaps.end());
+    std::vector<int64_t> pk;
+    std::vector<int32_t> row_piece;
+    if (!cut_pieces(ip, gaps, MQI_PIECE, pk, row_piece)) return fail(c, MQI_E_ARG, "too many Dij pieces");
+    m.n_piece = (int64_t) pk.size() - 1;
+    std::vector<int64_t> pb(m.n_piece + 1, 0);
+    for (int64_t i = 0; i < m.n_piece; ++i) pb[i + 1] = pb[i] + ((pk[i + 1] - pk[i] + 255) & ~(int64_t) 255);
+    if ((r = dalloc(c, &d_pk, pk.size())) || (r = dalloc(c, &m.pbase, std::max<int64_t>(m.n_piece, 1))) ||
+        (r = dalloc(c, &m.row_piece, rows + 1)) || (r = dalloc(c, &m.ppart, std::max<int64_t>(m.n_piece, 1))) ||
+        (r = dalloc(c, &m.pb, pb.size())) || (r = dalloc(c, &m.sval, std::max<int64_t>(pb.back(), 1))) ||
+        (r = dalloc(c, &m.sdcol, std::max<int64_t>(pb.back(), 1))) ||
+        (r = dalloc(c, &m.porder, std::max<int64_t>(m.n_piece, 1))))
+        return r;
+    HIPCHK(c, hipMemcpy(d_pk, pk.data(), sizeof(int64_t) * pk.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m.row_piece, row_piece.data(), sizeof(int32_t) * (rows + 1), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m.pb, pb.data(), sizeof(int64_t) * pb.size(), hipMemcpyHostToDevice));
+    KCHK(c, mqi_aux::piece_base(m.ccol, d_dcol, d_pk, m.n_piece, m.pbase, c->stream));
+    KCHK(c, mqi_aux::stage_pieces(m.val, d_dcol, d_pk, m.pb, m.n_piece, m.sval, m.sdcol, c->stream));
+    // processed in the order of their first columns: the pieces that run at
+    // the same time gather from the same stretch of x (7 % faster on the
+    // 5000-spot Dij than row order)
+    std::vector<int32_t> first(m.n_piece), order(m.n_piece);
+    if (m.n_piece) HIPCHK(c, hipMemcpy(first.data(), m.pbase, sizeof(int32_t) * m.n_piece, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < m.n_piece; ++i) order[i] = (int32_t) i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return first[a] < first[b]; });
+    if (m.n_piece) HIPCHK(c, hipMemcpy(m.porder, order.data(), sizeof(int32_t) * m.n_piece, hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 uint64_t
 nvox(const mqi_ctx* c) {
     return (uint64_t) c->nx * c->ny * c->nz;
@@ -352,8 +441,8 @@ upload_grid_common(mqi_ctx* c, int nx, int ny, int nz, const float* xe, const fl
 }
 
 size_t
-shmem_bytes(const mqi_ctx* c) {
-    return sizeof(float) * (3600 + (size_t) c->nx + 1 + c->ny + 1 + c->nz + 1 + c->bl_n_edges);
+shmem_bytes(const mqi_ctx* c, int threads) {
+    return mqi_shmem_bytes(c->nx, c->ny, c->nz, c->bl_n_edges, threads);
 }
 
 int
@@ -487,7 +576,7 @@ launch_bound(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dyn
     if (trace && (!P.trace_cnb || !P.trace_rec || !P.trace_n)) return (int) hipErrorInvalidValue;
     if ((uint64_t) blocks * threads * (uint64_t) P.stack_cap * MQI_STACK_FLOATS * sizeof(float) > stack_bytes)
         return (int) hipErrorInvalidValue;
-    const size_t sh = shmem_bytes(c);
+    const size_t sh = shmem_bytes(c, threads);
     if (dynamic) {
         const int e = mqi_aux::queue_init(P.queue, P.total, c->stream);
         if (e) return e;
@@ -524,7 +613,7 @@ check_tuning(mqi_ctx* c, const mqi_run_config& cfg) {
 int
 persistent_blocks(mqi_ctx* c, const mqi_run_config& cfg, int threads, int trace) {
     if (cfg.blocks > 0) return cfg.blocks;
-    const size_t sh  = shmem_bytes(c);
+    const size_t sh  = shmem_bytes(c, threads);
     int          occ = cfg.exact ? mqi_exact::occupancy(cfg.physics_debug, 1, trace, 0, threads, sh)
                                  : mqi_fast::occupancy(cfg.physics_debug, 1, trace, cfg.waves_per_simd, threads, sh);
     return c->n_cu * occ;
@@ -553,7 +642,7 @@ int
 graph_pass(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int blocks, int threads, double* ms) {
     bind_buffers(c, P);
     const int variant[6] = {cfg.physics_debug, cfg.exact, cfg.exact ? 0 : cfg.waves_per_simd, blocks, threads,
-                            (int) shmem_bytes(c)};
+                            (int) shmem_bytes(c, threads)};
     std::vector<unsigned char> key(sizeof P + sizeof variant);
     std::memcpy(key.data(), &P, sizeof P);
     std::memcpy(key.data() + sizeof P, variant, sizeof variant);
@@ -1496,6 +1585,11 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             KCHK(c, mqi_aux::renumber(m.col, m.nnz, d_rank, m.ccol, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             hipFree(d_rank);
+            // long rows (the Dij of pencil beams): D^T g over 16-bit columns
+            if (m.nnz >= (int64_t) MQI_SEG_SHORT * m.rows) {
+                if ((r = build_pieces(c, m, indptr))) return r;
+                dfree(m.ccol);
+            }
             int key_bits = 1;
             while (key_bits < 32 && (1ll << key_bits) < m.cols) ++key_bits;
             int32_t*  d_keys = nullptr;
@@ -1515,6 +1609,13 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             hipFree(d_pk2);
             hipFree(d_tmp);
             hipFree(d_cc);
+            // D w over 16-bit spot columns
+            if (m.rows <= 65536) {
+                if ((r = dalloc(c, &m.tcol16, m.nnz))) return r;
+                KCHK(c, mqi_aux::low16(m.tcol, m.nnz, m.tcol16, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                dfree(m.tcol);
+            }
         }
     }
     hipFree(d_max);
@@ -1716,15 +1817,21 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
     if (transpose) {
         // D^T g: rows of the spot-major matrix
         const csr_plan& p = m.plan;
-        const int32_t*  colp = m.col;
-        const float*    xp   = x;
-        if (m.ccol) {   // gather from the compacted x (12 MB instead of 33.5 MB on the 5000-spot plan)
+        if (m.sval) {   // pieces over 16-bit column deltas of the compacted x
             KCHK(c, mqi_aux::gather_x(x, m.nzvox, m.n_nz, m.xc, st));
-            colp = m.ccol;
-            xp   = m.xc;
+            KCHK(c, mqi_aux::pieces_staged(m.sval, m.sdcol, m.xc, m.pb, m.pbase, m.porder, m.n_piece, m.row_piece,
+                                           m.rows, m.ppart, y, y_double, alpha, accumulate, st));
+        } else {
+            const int32_t* colp = m.col;
+            const float*   xp   = x;
+            if (m.ccol) {   // gather from the compacted x (12 MB instead of 33.5 MB on the 5000-spot plan)
+                KCHK(c, mqi_aux::gather_x(x, m.nzvox, m.n_nz, m.xc, st));
+                colp = m.ccol;
+                xp   = m.xc;
+            }
+            KCHK(c, mqi_aux::csr_adaptive(m.indptr, colp, m.val, xp, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
+                                          p.n_split, p.part, y, y_double, alpha, accumulate, st));
         }
-        KCHK(c, mqi_aux::csr_adaptive(m.indptr, colp, m.val, xp, p.seg_k, p.seg_info, p.n_seg, p.split_seg, p.n_split,
-                                      p.part, y, y_double, alpha, accumulate, st));
     } else if (m.nnz == 0) {
         // D w of an empty matrix: zeros (or y unchanged)
         if (!accumulate) {
@@ -1736,8 +1843,12 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
     } else {
         // D w: rows of the voxel-major copy
         const csr_plan& p = m.tplan;
-        KCHK(c, mqi_aux::csr_adaptive(m.tindptr, m.tcol, m.tval, x, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
-                                      p.n_split, p.part, y, y_double, alpha, accumulate, st));
+        if (m.tcol16)
+            KCHK(c, mqi_aux::csr_adaptive16(m.tindptr, m.tcol16, m.tval, x, p.seg_k, p.seg_info, p.n_seg,
+                                            p.split_seg, p.n_split, p.part, y, y_double, alpha, accumulate, st));
+        else
+            KCHK(c, mqi_aux::csr_adaptive(m.tindptr, m.tcol, m.tval, x, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
+                                          p.n_split, p.part, y, y_double, alpha, accumulate, st));
     }
     HIPCHK(c, hipEventRecord(c->ev_mv1, st));
     c->matvec_pending = true;
@@ -1771,6 +1882,28 @@ mqi_math_probe(mqi_ctx* c, int fn, const double* x, const double* y, int64_t n, 
     hipFree(dx);
     hipFree(dy);
     hipFree(dout);
+    return 0;
+}
+
+int
+mqi_atomic_probe(mqi_ctx* c, int64_t n_slots, int64_t n_updates, double* ms) {
+    if (!c || !ms || n_slots <= 0 || n_updates <= 0 || n_slots > (int64_t(1) << 32)) return MQI_E_ARG;
+    hipSetDevice(c->device);
+    double* d = nullptr;
+    int     r = dalloc(c, &d, (size_t) n_slots);
+    if (r) return r;
+    int e = mqi_aux::fill_u64(d, 0ull, (uint64_t) n_slots, c->stream);
+    if (!e) e = mqi_aux::atomic_probe(d, (uint64_t) n_slots, (uint64_t) n_updates, c->stream);   // warm-up
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    if (!e) e = (int) hipEventRecord(e0, c->stream);
+    if (!e) e = mqi_aux::atomic_probe(d, (uint64_t) n_slots, (uint64_t) n_updates, c->stream);
+    if (!e) e = (int) hipEventRecord(e1, c->stream);
+    if (!e) e = (int) hipEventSynchronize(e1);
+    float t = 0;
+    if (!e) e = (int) hipEventElapsedTime(&t, e0, e1);
+    hipFree(d);
+    if (e) return fail(c, MQI_E_HIP, "atomic probe: %s", hipGetErrorString((hipError_t) e));
+    *ms = t;
     return 0;
 }
 

@@ -13,6 +13,8 @@ int queue_init(unsigned long long* q, uint64_t total, hipStream_t s);
 int fill_u32(void* p, uint32_t v, uint64_t n, hipStream_t s);
 int hu_to_density(const int16_t* hu, float* rho, uint64_t n, float scale, hipStream_t s);
 int math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t s);
+// n_updates FP64 atomic adds to hashed slots of a (n_slots doubles)
+int atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, hipStream_t s);
 int stat_reduce(const double* s1, const double* s2, uint64_t n, double N, float thr, float* d_max, double* d_sum,
                 unsigned long long* d_cnt, hipStream_t s);
 int scale(double* p, uint64_t n, double w, hipStream_t s);
@@ -36,6 +38,31 @@ int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tm
 int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
                  const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
                  void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
+// the same over 16-bit columns (D w of a Dij with at most 65536 spots)
+int csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
+                   const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+                   void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
+// D^T g over 16-bit column deltas (mqi_aux.hip): the deltas of each row's
+// columns (0 at row starts and where a row skips more than 65535 columns,
+// those positions listed in gaps), the column before each piece's first
+// delta, the pieces restaged lane-major in whole 256-nonzero batches (pb:
+// staged starts, n_p + 1 entries), and the product: one wave per piece into
+// part[piece] in the order order[] gives, then each row's pieces summed in
+// a fixed order.
+#define MQI_PIECE 4096          // nonzeros per piece
+#define MQI_PIECE_WAVES 8       // waves per workgroup of the product
+#define MQI_PIECE_BATCHES 8     // 256-nonzero batches a wave has in flight
+int deltas16(const int64_t* indptr, const int32_t* col, int64_t rows, int64_t nnz, uint16_t* dcol,
+             unsigned long long* n_gaps, int64_t* gaps, int64_t cap, hipStream_t s);
+int piece_base(const int32_t* col, const uint16_t* dcol, const int64_t* pk, int64_t n_p, int32_t* pbase,
+               hipStream_t s);
+int stage_pieces(const float* val, const uint16_t* dcol, const int64_t* pk, const int64_t* pb, int64_t n_p,
+                 float* sval, uint16_t* sdcol, hipStream_t s);
+int pieces_staged(const float* sval, const uint16_t* sdcol, const float* x, const int64_t* pb, const int32_t* pbase,
+                  const int32_t* order, int64_t n_p, const int32_t* row_piece, int64_t rows, double* part, void* y,
+                  int y_is_double, double alpha, int accumulate, hipStream_t s);
+// low 16 bits of int32 columns (D w over at most 65536 spots)
+int low16(const int32_t* col, int64_t nnz, uint16_t* out, hipStream_t s);
 // D^T g on compacted x: xc = x[nzvox]; ccol = rank[col]
 int gather_x(const float* x, const int32_t* nzvox, int64_t n, float* xc, hipStream_t s);
 int renumber(const int32_t* col, int64_t nnz, const int32_t* rank, int32_t* ccol, hipStream_t s);

@@ -103,6 +103,35 @@ math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hip
     return (int) hipGetLastError();
 }
 
+// ------------------------------------------------------------------ atomic probe
+// n_updates FP64 atomic adds of 1.0 to slots picked by a 64-bit hash of the
+// update index over n_slots: the scoring pattern of the transport (random
+// voxels, device-scope adds whose results are not read) without the rest of
+// the step, to measure the part's throughput for such atomics.
+__global__ void k_atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates) {
+    // slot = the murmur3 32-bit finaliser of i scaled to [0, n_slots) by a
+    // high multiply (n_slots <= 2^32): a few integer operations per update,
+    // so the probe measures the atomic stream, not a 64-bit modulo
+    for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n_updates;
+         i += (uint64_t) gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t) i;
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        atomicAdd(a + (((uint64_t) h * n_slots) >> 32), 1.0);
+    }
+}
+
+int
+atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, hipStream_t s) {
+    if (n_slots == 0 || n_updates == 0) return 0;
+    if (n_slots > (1ull << 32)) return (int) hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_atomic_probe, dim3(65536), dim3(256), 0, s, a, n_slots, n_updates);
+    return (int) hipGetLastError();
+}
+
 // ------------------------------------------------------------------ statistics
 // calculate_standard_deviation (mqi_variables.hpp:21-48) fused with the
 // reduction of calculate_stat (mqi_tps_env.hpp:1396-1425):
@@ -302,12 +331,19 @@ segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, i
 // Every sum runs in a fixed order, so results do not depend on scheduling.
 // seg_info[s] = first row | (partial << 31); seg_k[s] = first nonzero;
 // entry n_seg of both is the sentinel (rows, nnz).
-typedef float   f32x4 __attribute__((ext_vector_type(4)));
-typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef float    f32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t  i32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
 
-template<typename TY>
+// four columns in one load: int32 columns, or 16-bit ones (D w over at most
+// 65536 spots)
+template<typename TC> struct col4;
+template<> struct col4<int32_t> { typedef i32x4 type; };
+template<> struct col4<uint16_t> { typedef u16x4 type; };
+
+template<typename TY, typename TC>
 __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__ indptr,
-                                                     const int32_t* __restrict__ col,
+                                                     const TC* __restrict__ col,
                                                      const float* __restrict__ val, const float* __restrict__ x,
                                                      const int64_t* __restrict__ seg_k,
                                                      const uint32_t* __restrict__ seg_info, int64_t n_seg,
@@ -326,12 +362,13 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             if (ka > ke) ka = ke;
             if (lane < ka - kb) acc += (double) val[kb + lane] * (double) x[col[kb + lane]];
             const int64_t n4 = (ke - ka) >> 2;
+            typedef typename col4<TC>::type c4t;
             const f32x4*  v4 = reinterpret_cast<const f32x4*>(val + ka);
-            const i32x4*  c4 = reinterpret_cast<const i32x4*>(col + ka);
+            const c4t*    c4 = reinterpret_cast<const c4t*>(col + ka);
             int64_t       i  = lane;
             for (; i + 192 < n4; i += 256) {
                 f32x4 v[4];
-                i32x4 c[4];
+                c4t   c[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     v[u] = __builtin_nontemporal_load(v4 + i + 64 * u);
@@ -347,7 +384,7 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             }
             for (; i < n4; i += 64) {
                 const f32x4 v = __builtin_nontemporal_load(v4 + i);
-                const i32x4 c = __builtin_nontemporal_load(c4 + i);
+                const c4t   c = __builtin_nontemporal_load(c4 + i);
                 acc += (double) v.x * (double) x[c.x];
                 acc += (double) v.y * (double) x[c.y];
                 acc += (double) v.z * (double) x[c.z];
@@ -363,7 +400,7 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             int64_t k = kb + lane;
             for (; k + 192 < ke; k += 256) {
                 float   v[4];
-                int32_t cc[4];
+                int32_t cc[4];   // TC widened
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     v[u]  = val[k + 64 * u];
@@ -414,18 +451,19 @@ __global__ void k_csr_split_rows(const uint32_t* __restrict__ seg_info, const in
     y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
 }
 
+template<typename TC>
 int
-csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
-             const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-             void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+csr_adaptive_t(const int64_t* indptr, const TC* col, const float* val, const float* x, const int64_t* seg_k,
+               const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+               void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
     if (n_seg <= 0) return 0;
     const int blocks = (int) (n_seg < 262144 ? n_seg : 262144);
     if (y_is_double)
-        hipLaunchKernelGGL(k_csr_adaptive<double>, dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k, seg_info,
-                           n_seg, part, (double*) y, alpha, accumulate);
+        hipLaunchKernelGGL((k_csr_adaptive<double, TC>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k,
+                           seg_info, n_seg, part, (double*) y, alpha, accumulate);
     else
-        hipLaunchKernelGGL(k_csr_adaptive<float>, dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k, seg_info,
-                           n_seg, part, (float*) y, alpha, accumulate);
+        hipLaunchKernelGGL((k_csr_adaptive<float, TC>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k,
+                           seg_info, n_seg, part, (float*) y, alpha, accumulate);
     if (n_split > 0) {
         const int b = (int) ((n_split + 255) / 256);
         if (y_is_double)
@@ -438,12 +476,252 @@ csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const 
     return (int) hipGetLastError();
 }
 
+int
+csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
+             const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+             void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+    return csr_adaptive_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, y, y_is_double,
+                          alpha, accumulate, s);
+}
+int
+csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
+               const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+               void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+    return csr_adaptive_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, y, y_is_double,
+                          alpha, accumulate, s);
+}
+
+// ------------------------------------------------------------------ D^T g over 16-bit column deltas
+// Index compression of the CSR stream (delta coding of the column indices,
+// after K. Kourtis, G. Goumas and N. Koziris, "Optimizing sparse
+// matrix-vector multiplication using index and value compression", CF'08):
+// a nonzero keeps its value and the difference of its column to the
+// previous nonzero of its row in 16 bits, 6 bytes instead of 8.  The
+// nonzeros are cut into pieces of <= MQI_PIECE entries of one row, also
+// where a row skips more than 65535 columns; a piece keeps the column its
+// first delta adds to (pbase).  For the product the pieces are restaged:
+// padded with zero values and zero deltas to whole batches of 256 nonzeros,
+// each batch lane-major (the 16-byte load of lane l holds the batch's
+// nonzeros l, l + 64, l + 128 and l + 192), so that each of the four gathers
+// of a batch reads the columns of 64 consecutive nonzeros: few distinct
+// cache lines per gather instruction instead of every fourth column of 256.
+// One wave per piece rebuilds the columns with wave prefix sums of the
+// deltas and forms the FP64 sum of the piece into part[piece]; a second pass
+// adds each row's pieces in a fixed order, so results do not depend on
+// scheduling or on the order the pieces run in.  (Offsets from each batch's
+// first column instead of deltas need no prefix sums, but the sparse halo of
+// a spot spans more than 65535 columns in 256 nonzeros so often that the
+// cuts it needs made the pieces tiny: 13x slower on the 5000-spot Dij.)
+
+// dcol[k] = col[k] - col[k - 1] where that is in [1, 65535], else 0; the
+// positions with a larger step are listed in gaps[] (count in *n_gaps, at
+// most cap written).  Row starts are cleared afterwards (k_row_starts).
+__global__ void k_deltas(const int32_t* __restrict__ col, int64_t nnz, uint16_t* __restrict__ dcol,
+                         unsigned long long* __restrict__ n_gaps, int64_t* __restrict__ gaps, int64_t cap) {
+    for (int64_t k = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t) gridDim.x * blockDim.x) {
+        const int64_t d = k > 0 ? (int64_t) col[k] - (int64_t) col[k - 1] : 0;
+        const bool    ok = d >= 1 && d <= 65535;
+        dcol[k]          = ok ? (uint16_t) d : (uint16_t) 0;
+        if (!ok && d > 65535) {
+            const unsigned long long at = atomicAdd(n_gaps, 1ull);
+            if ((int64_t) at < cap) gaps[at] = k;
+        }
+    }
+}
+
+__global__ void k_row_starts(const int64_t* __restrict__ indptr, int64_t rows, uint16_t* __restrict__ dcol) {
+    const int64_t r = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows && indptr[r + 1] > indptr[r]) dcol[indptr[r]] = 0;
+}
+
+// pbase[p] = col[pk[p]] - dcol[pk[p]]: the column the piece's first delta
+// adds to (0 for the empty piece of an empty row)
+__global__ void k_piece_base(const int32_t* __restrict__ col, const uint16_t* __restrict__ dcol,
+                             const int64_t* __restrict__ pk, int64_t n_p, int32_t* __restrict__ pbase) {
+    const int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n_p) pbase[p] = pk[p] < pk[p + 1] ? col[pk[p]] - (int32_t) dcol[pk[p]] : 0;
+}
+
+// pb[p]: the piece's first staged slot (a multiple of 256); pb[n_p]: total
+__global__ void k_stage_pieces(const float* __restrict__ val, const uint16_t* __restrict__ dcol,
+                               const int64_t* __restrict__ pk, const int64_t* __restrict__ pb, int64_t n_p,
+                               float* __restrict__ sval, uint16_t* __restrict__ sdcol) {
+    for (int64_t p = blockIdx.x; p < n_p; p += gridDim.x) {
+        const int64_t kb = pk[p], len = pk[p + 1] - kb, base = pb[p], padded = pb[p + 1] - base;
+        for (int64_t i = threadIdx.x; i < padded; i += blockDim.x) {
+            const int64_t j   = i & 255;
+            const int64_t dst = base + (i & ~(int64_t) 255) + ((j & 63) << 2) + (j >> 6);
+            sval[dst]         = i < len ? val[kb + i] : 0.0f;
+            sdcol[dst]        = i < len ? dcol[kb + i] : (uint16_t) 0;
+        }
+    }
+}
+
+// inclusive prefix sum over the wave: DPP row shifts within the four rows
+// of 16 lanes, then the row broadcasts of lanes 15 and 31
+MQI_DI uint32_t wave_scan_dpp(uint32_t x) {
+    x += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0x111, 0xf, 0xf, false);   // row_shr:1
+    x += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0x112, 0xf, 0xf, false);   // row_shr:2
+    x += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0x114, 0xf, 0xf, false);   // row_shr:4
+    x += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0x118, 0xf, 0xf, false);   // row_shr:8
+    x += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    x += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) x, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return x;
+}
+
+// One wave per piece, MQI_PIECE_BATCHES batches (2048 nonzeros) in flight;
+// the pieces run in the order order[] gives (their first columns ascending:
+// pieces that run at the same time gather from the same stretch of x).
+__global__ void __launch_bounds__(64 * MQI_PIECE_WAVES)
+    k_pieces_staged(const float* __restrict__ sval, const uint16_t* __restrict__ sdcol, const float* __restrict__ x,
+                    const int64_t* __restrict__ pb, const int32_t* __restrict__ pbase,
+                    const int32_t* __restrict__ order, int64_t n_p, double* __restrict__ part) {
+    constexpr int U    = MQI_PIECE_BATCHES;
+    const int     lane = threadIdx.x & 63;
+    const int64_t w    = (int64_t) blockIdx.x * MQI_PIECE_WAVES + (threadIdx.x >> 6);
+    const int64_t nw   = (int64_t) gridDim.x * MQI_PIECE_WAVES;
+    for (int64_t i = w; i < n_p; i += nw) {
+        const int64_t p  = order[i];
+        const int64_t b0 = pb[p] >> 8, nb = (pb[p + 1] >> 8) - b0;
+        const f32x4*  v4 = reinterpret_cast<const f32x4*>(sval) + b0 * 64;
+        const u16x4*  d4 = reinterpret_cast<const u16x4*>(sdcol) + b0 * 64;
+        uint32_t      run = (uint32_t) pbase[p];
+        double        acc = 0.0;
+        for (int64_t b = 0; b < nb; b += U) {
+            f32x4 v[U];
+            u16x4 d[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (b + u < nb) {
+                    v[u] = __builtin_nontemporal_load(v4 + (b + u) * 64 + lane);
+                    d[u] = __builtin_nontemporal_load(d4 + (b + u) * 64 + lane);
+                } else {
+                    v[u] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
+                    d[u] = u16x4{ 0, 0, 0, 0 };
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // quarter q of the batch (nonzeros 64 q .. 64 q + 63, one per
+                // lane): its columns are a prefix sum over the wave
+                const uint32_t s0 = wave_scan_dpp(d[u].x);
+                const uint32_t t0 = (uint32_t) __builtin_amdgcn_readlane((int) s0, 63);
+                const uint32_t s1 = wave_scan_dpp(d[u].y);
+                const uint32_t t1 = (uint32_t) __builtin_amdgcn_readlane((int) s1, 63);
+                const uint32_t s2 = wave_scan_dpp(d[u].z);
+                const uint32_t t2 = (uint32_t) __builtin_amdgcn_readlane((int) s2, 63);
+                const uint32_t s3 = wave_scan_dpp(d[u].w);
+                const uint32_t t3 = (uint32_t) __builtin_amdgcn_readlane((int) s3, 63);
+                acc += (double) v[u].x * (double) x[run + s0];
+                acc += (double) v[u].y * (double) x[run + t0 + s1];
+                acc += (double) v[u].z * (double) x[run + t0 + t1 + s2];
+                acc += (double) v[u].w * (double) x[run + t0 + t1 + t2 + s3];
+                run += t0 + t1 + t2 + t3;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) part[p] = acc;
+    }
+}
+
+// y[r] = alpha * (the pieces row_piece[r] .. row_piece[r + 1] - 1) (+ y[r]):
+// one wave per row, lane l takes pieces q0 + l + 64 k, the wave adds them in
+// a fixed tree order
+template<typename TY>
+__global__ void k_piece_rows(const int32_t* __restrict__ row_piece, int64_t rows, const double* __restrict__ part,
+                             TY* __restrict__ y, double alpha, int accumulate) {
+    const int64_t r    = (int64_t) blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int     lane = threadIdx.x & 63;
+    if (r >= rows) return;
+    const int32_t q0 = row_piece[r], q1 = row_piece[r + 1];
+    double        acc = 0.0;
+    for (int32_t q = q0 + lane; q < q1; q += 64) acc += part[q];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+        const double out = alpha * acc;
+        y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+    }
+}
+
+int
+deltas16(const int64_t* indptr, const int32_t* col, int64_t rows, int64_t nnz, uint16_t* dcol,
+         unsigned long long* n_gaps, int64_t* gaps, int64_t cap, hipStream_t s) {
+    if (nnz <= 0) return 0;
+    int e = fill_u64(n_gaps, 0ull, 1, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_deltas, dim3(grid_for((uint64_t) nnz)), dim3(256), 0, s, col, nnz, dcol, n_gaps, gaps, cap);
+    if (rows > 0)
+        hipLaunchKernelGGL(k_row_starts, dim3((unsigned) ((rows + 255) / 256)), dim3(256), 0, s, indptr, rows, dcol);
+    return (int) hipGetLastError();
+}
+
+int
+piece_base(const int32_t* col, const uint16_t* dcol, const int64_t* pk, int64_t n_p, int32_t* pbase, hipStream_t s) {
+    if (n_p <= 0) return 0;
+    hipLaunchKernelGGL(k_piece_base, dim3((unsigned) ((n_p + 255) / 256)), dim3(256), 0, s, col, dcol, pk, n_p, pbase);
+    return (int) hipGetLastError();
+}
+
+int
+stage_pieces(const float* val, const uint16_t* dcol, const int64_t* pk, const int64_t* pb, int64_t n_p, float* sval,
+             uint16_t* sdcol, hipStream_t s) {
+    if (n_p <= 0) return 0;
+    hipLaunchKernelGGL(k_stage_pieces, dim3((unsigned) (n_p < 65536 ? n_p : 65536)), dim3(256), 0, s, val, dcol, pk, pb,
+                       n_p, sval, sdcol);
+    return (int) hipGetLastError();
+}
+
+int
+pieces_staged(const float* sval, const uint16_t* sdcol, const float* x, const int64_t* pb, const int32_t* pbase,
+              const int32_t* order, int64_t n_p, const int32_t* row_piece, int64_t rows, double* part, void* y,
+              int y_is_double, double alpha, int accumulate, hipStream_t s) {
+    if (n_p > 0) {
+        const int64_t b = (n_p + MQI_PIECE_WAVES - 1) / MQI_PIECE_WAVES;
+        hipLaunchKernelGGL(k_pieces_staged, dim3((unsigned) (b < 65536 ? b : 65536)), dim3(64 * MQI_PIECE_WAVES), 0,
+                           s, sval, sdcol, x, pb, pbase, order, n_p, part);
+    }
+    if (rows > 0) {
+        const unsigned b = (unsigned) ((rows + 3) / 4);
+        if (y_is_double)
+            hipLaunchKernelGGL(k_piece_rows<double>, dim3(b), dim3(256), 0, s, row_piece, rows, part, (double*) y,
+                               alpha, accumulate);
+        else
+            hipLaunchKernelGGL(k_piece_rows<float>, dim3(b), dim3(256), 0, s, row_piece, rows, part, (float*) y,
+                               alpha, accumulate);
+    }
+    return (int) hipGetLastError();
+}
+
+// low 16 bits of int32 columns (D w over at most 65536 spots)
+__global__ void k_low16(const int32_t* __restrict__ col, int64_t nnz, uint16_t* __restrict__ out) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t) gridDim.x * blockDim.x)
+        out[i] = (uint16_t) (col[i] & 0xffff);
+}
+
+int
+low16(const int32_t* col, int64_t nnz, uint16_t* out, hipStream_t s) {
+    if (nnz <= 0) return 0;
+    hipLaunchKernelGGL(k_low16, dim3(grid_for((uint64_t) nnz)), dim3(256), 0, s, col, nnz, out);
+    return (int) hipGetLastError();
+}
+
 // Compacted x for D^T g: xc[i] = x[nzvox[i]] (nzvox: the voxels that appear
 // in some row, ascending); the rows' columns renumbered into that list.
+// four entries per thread: 16-byte index loads and stores
 __global__ void k_gather_x(const float* __restrict__ x, const int32_t* __restrict__ nzvox, int64_t n,
                            float* __restrict__ xc) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
-        xc[i] = x[nzvox[i]];
+    const int64_t n4 = n >> 2;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t) gridDim.x * blockDim.x) {
+        const i32x4 v = __builtin_nontemporal_load(reinterpret_cast<const i32x4*>(nzvox) + i);
+        f32x4       o;
+        o.x = x[v.x];
+        o.y = x[v.y];
+        o.z = x[v.z];
+        o.w = x[v.w];
+        __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(xc) + i);
+    }
+    const int64_t t = (n4 << 2) + (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && t < n) xc[t] = x[nzvox[t]];
 }
 __global__ void k_renumber(const int32_t* __restrict__ col, int64_t nnz, const int32_t* __restrict__ rank,
                            int32_t* __restrict__ ccol) {
@@ -454,7 +732,7 @@ __global__ void k_renumber(const int32_t* __restrict__ col, int64_t nnz, const i
 int
 gather_x(const float* x, const int32_t* nzvox, int64_t n, float* xc, hipStream_t s) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_gather_x, dim3(grid_for((uint64_t) n)), dim3(256), 0, s, x, nzvox, n, xc);
+    hipLaunchKernelGGL(k_gather_x, dim3(grid_for((uint64_t) (n + 3) / 4)), dim3(256), 0, s, x, nzvox, n, xc);
     return (int) hipGetLastError();
 }
 int

@@ -511,9 +511,30 @@ MQI_DI void rotate_direction(v3& d, float theta, float phi) {
 }
 
 // ------------------------------------------------------------------ random numbers
+// The verification build draws every number from one lazily refilled stream
+// in the order the CPU restatement does.  The production build draws the four
+// numbers every step takes (mean free path, straggling and scattering
+// normals, azimuth) from one Philox block made for all lanes at the start of
+// the step, and the first three of an interaction (process, first delta-ray
+// trial) from one block made where the step ends in an interaction; the lazy
+// stream serves the rest (later delta-ray trials, nuclear interactions).
+// With the lazy stream alone, every draw site refilled for some lane of the
+// wave (8.3 block evaluations per wave step on the 2000-spot CT plan).
 struct rng_t {
     mqi_rng::stream s;
-    MQI_DI float    uniform() { return s.uniform(); }
+#if !MQI_EXACT
+    uint32_t step_blk, int_blk;   // blocks of the two per-step streams used so far
+    float    st_u0, st_u1, st_z0, st_z1;   // the step's two uniforms and two normals
+    float    it_u0, it_u1, it_u2;          // the interaction's three uniforms
+#endif
+    MQI_DI void init(uint64_t seed, uint32_t history, uint32_t spot, uint32_t scenario, uint32_t purpose) {
+        s.init(seed, history, spot, scenario, purpose);
+#if !MQI_EXACT
+        step_blk = 0;
+        int_blk  = 0;
+#endif
+    }
+    MQI_DI float uniform() { return s.uniform(); }
     // standard normal: the exact build evaluates Box-Muller as the CPU
     // restatement does; the production build in float with the device library
     MQI_DI float gauss() {
@@ -537,6 +558,59 @@ struct rng_t {
     MQI_DI float normal(float avg, float sig) {
         float z = gauss();
         return z * sig + avg;
+    }
+
+    // the draws of a step: step_uniform(0) the mean free path, step_normal(0)
+    // the energy straggling, step_normal(1) the scattering angle,
+    // step_uniform(1) the azimuth, in that order
+    MQI_DI void begin_step() {
+#if !MQI_EXACT
+        const mqi_rng::u32x4 w = s.block_of(mqi_rng::STREAM_STEP, step_blk++);
+        st_u0                  = mqi_rng::stream::to_uniform(w.v[0]);
+        st_u1                  = mqi_rng::stream::to_uniform(w.v[3]);
+        // Box-Muller on the hardware log2, square root, sine and cosine (the
+        // last two take revolutions: sin(2 pi u))
+        const float u1 = mqi_rng::stream::to_uniform(w.v[1]);
+        const float u2 = mqi_rng::stream::to_uniform(w.v[2]);
+        const float r  = __builtin_sqrtf(-2.0f * 0.693147180559945309f * __builtin_amdgcn_logf(u1));
+        st_z0          = r * __builtin_amdgcn_cosf(u2);
+        st_z1          = r * __builtin_amdgcn_sinf(u2);
+#endif
+    }
+    MQI_DI float step_uniform(int k) {
+#if MQI_EXACT
+        (void) k;
+        return uniform();
+#else
+        return k == 0 ? st_u0 : st_u1;
+#endif
+    }
+    MQI_DI float step_normal(int k, float avg, float sig) {
+#if MQI_EXACT
+        (void) k;
+        return normal(avg, sig);
+#else
+        return (k == 0 ? st_z0 : st_z1) * sig + avg;
+#endif
+    }
+    // the draws of an interaction: int_uniform(0) the process,
+    // int_uniform(1) and (2) the first delta-ray trial; later trials and the
+    // nuclear interactions draw uniform()
+    MQI_DI void begin_interaction() {
+#if !MQI_EXACT
+        const mqi_rng::u32x4 w = s.block_of(mqi_rng::STREAM_INTERACTION, int_blk++);
+        it_u0                  = mqi_rng::stream::to_uniform(w.v[0]);
+        it_u1                  = mqi_rng::stream::to_uniform(w.v[1]);
+        it_u2                  = mqi_rng::stream::to_uniform(w.v[2]);
+#endif
+    }
+    MQI_DI float int_uniform(int k) {
+#if MQI_EXACT
+        (void) k;
+        return uniform();
+#else
+        return k == 0 ? it_u0 : (k == 1 ? it_u1 : it_u2);
+#endif
     }
 };
 

@@ -30,6 +30,21 @@ enum {
     MQI_LOOPS_STAT  = MQI_LOOPS_REFERENCE_STAT     // (mqi_transport.hpp:343-364)
 };
 
+// Scorer slots whose deposits a lane sums per voxel before one atomic add
+// (the rest are added per step), and where those sums sit in the dynamic LDS
+// of the transport kernel: after the six 600-entry tables, the three edge
+// arrays and the beamline edges, at an 8-byte boundary, MQI_DEFER doubles
+// per thread.
+#define MQI_DEFER 4
+#define MQI_LDS_TABLE_FLOATS 3600
+__host__ __device__ inline uint32_t mqi_pend_offset(int nx, int ny, int nz, int bl_edges) {
+    const uint32_t f = MQI_LDS_TABLE_FLOATS + (uint32_t) (nx + 1 + ny + 1 + nz + 1 + bl_edges);
+    return (f + 1u) & ~1u;
+}
+__host__ __device__ inline size_t mqi_shmem_bytes(int nx, int ny, int nz, int bl_edges, int threads) {
+    return sizeof(float) * (size_t) mqi_pend_offset(nx, ny, nz, bl_edges) + sizeof(double) * MQI_DEFER * (size_t) threads;
+}
+
 // floats per secondary-stack entry: ke0, pos[3], dir[3], ke1, dE, and the
 // beamline child whose transform still has to place it (-1: placed)
 #define MQI_STACK_FLOATS 10

@@ -51,7 +51,9 @@ philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
     return c;
 }
 
-enum purpose_t : uint32_t { STREAM_TRANSPORT = 0, STREAM_VERTEX = 1 };
+// STREAM_STEP and STREAM_INTERACTION: the production transport's
+// block-per-step and block-per-interaction draws (mqi_device.hpp rng_t)
+enum purpose_t : uint32_t { STREAM_TRANSPORT = 0, STREAM_VERTEX = 1, STREAM_STEP = 2, STREAM_INTERACTION = 3 };
 
 // Per-history stream.  uniform() returns k * 2^-24, k in [1, 2^24], i.e. a
 // float in (0, 1] like curand_uniform; normal() is Box-Muller on a pair of
@@ -107,7 +109,25 @@ struct stream {
 
     MQI_HD float
     uniform() {
-        return (float) ((next_u32() >> 8) + 1u) * (1.0f / 16777216.0f);
+        return to_uniform(next_u32());
+    }
+
+    // k * 2^-24, k in [1, 2^24]: a float in (0, 1] like curand_uniform
+    MQI_HD static float
+    to_uniform(uint32_t w) {
+        return (float) ((w >> 8) + 1u) * (1.0f / 16777216.0f);
+    }
+
+    // block `b` of this history's stream for another purpose (same key and
+    // history, spot, scenario)
+    MQI_HD u32x4
+    block_of(uint32_t purpose, uint32_t b) const {
+        u32x4 c;
+        c.v[0] = b;
+        c.v[1] = c1;
+        c.v[2] = c2;
+        c.v[3] = (c3 & ~0xffu) | purpose;
+        return philox4x32_10(c, key0, key1);
     }
 
     // Standard normal variate in double (Box-Muller pair, second one cached).

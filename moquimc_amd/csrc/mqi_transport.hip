@@ -104,19 +104,43 @@ MQI_DI float energy_loss(ctx& c, const rel_q& rel, float rho, float step_length,
     float r = intpl(rel.Ek, x0, x1, R[n], R[n + 1]);
     if (r < L_w) return rel.Ek;
     r -= L_w;
+#if MQI_EXACT
     for (;;) {
         if (r >= R[n]) break;
         if (n == 0) break;
         --n;
         if (n == 0) break;
     }
+#else
+    // The walk above ends at the largest n in [1, n0] with R[n] <= r (0 if
+    // none), R being non-decreasing.  Walking from the current bin takes one
+    // LDS round trip per 0.5 MeV lost, the longest lane setting the wave's
+    // count (8.3 per step on the 2000-spot CT plan); here the walk starts at
+    // the bin the range slope of the current bin predicts, and goes up or
+    // down from there to the same n.
+    {
+        const int n0 = n;
+        const float slope = R[n0 + 1] - R[n0];   // mm per 0.5 MeV
+        int         k     = slope > 0.0f ? (int) (L_w / slope) : 0;
+        int         m     = n0 - (k < n0 ? k : n0);
+        if (m >= 1 && R[m] > r) {
+            do {
+                --m;
+            } while (m >= 1 && R[m] > r);
+            if (m < 1) m = 0;
+        } else {
+            while (m < n0 && R[m + 1] <= r) ++m;
+        }
+        n = (uint16_t) m;
+    }
+#endif
     x0            = 0.1f + n * 0.5f;
     x1            = x0 + 0.5f;
     float dE_mean = rel.Ek - intpl(r, R[n], R[n + 1], x0, x1);
     float Te      = (rel.Te_max >= 0.08511) ? 0.08511 : rel.Te_max;
     float O_sq    = units::two_pi_re2_mc2_h2o * rho / units::water_density * L_w;
     O_sq *= Te / rel.beta_sq * (1.0 - 0.5 * rel.beta_sq);
-    float ret = c.rng.normal(dE_mean, MQ_SQRTF(O_sq));
+    float ret = c.rng.step_normal(0, dE_mean, MQ_SQRTF(O_sq));
     if (ret < 0) ret *= -1.0;
     return ret;
 }
@@ -135,9 +159,9 @@ MQI_DI void along_step(ctx& c, trk& t, float len, float rho, float sp) {
     float X0    = radiation_length<DBG>(rho);
     float th_sq = ((13.9f / P) * (13.9f / P) / rel.beta_sq) * len / X0;
     float th    = MQ_SQRTF(th_sq);
-    th          = c.rng.normal(0, MQ_SQRTF(2.0f) * th);
+    th          = c.rng.step_normal(1, 0, MQ_SQRTF(2.0f) * th);
     if (th < 0) th *= -1.0;
-    float phi = 2.0 * 3.14159265358979323846 * c.rng.uniform();
+    float phi = 2.0 * 3.14159265358979323846 * c.rng.step_uniform(1);
     rotate_direction(t.d1, th, phi);
     t.dE += dE * r;
     update_pos(t, r * len);
@@ -203,10 +227,11 @@ MQI_DI void post_ion(ctx& c, trk& t) {
     // acceptance probability is above 1/2 for any finite energy; the bound
     // only ends the loop for non-finite input (which the reference would spin on)
     for (int it = 0; it < 4096; ++it) {
-        n  = c.rng.uniform();
+        n  = it == 0 ? c.rng.int_uniform(1) : c.rng.uniform();
         Te = T_cut * rel.Te_max;
         Te /= ((1.0 - n) * rel.Te_max + n * T_cut);
-        if (c.rng.uniform() < 1.0 - rel.beta_sq * Te / rel.Te_max + Te * Te / (2.0 * rel.Et_sq)) break;
+        const float acc = it == 0 ? c.rng.int_uniform(2) : c.rng.uniform();
+        if (acc < 1.0 - rel.beta_sq * Te / rel.Te_max + Te * Te / (2.0 * rel.Et_sq)) break;
     }
     if (DBG) {
         push_placed(c, 0.0f, t.p0, t.d0, 0.0f, Te);
@@ -388,7 +413,8 @@ MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b, float& sp) {
     float cs_sum  = first ? cs1_sum : cs2_sum;
     float cs0 = first ? cs1[0] : cs2[0], csa = first ? cs1[1] : cs2[1], csb = first ? cs1[2] : cs2[2],
           csc = first ? cs1[3] : cs2[3];
-    float prob       = c.rng.uniform();
+    c.rng.begin_step();
+    float prob       = c.rng.step_uniform(0);
     float mfp        = -1.0f * MQ_LOGF(prob) / cs_sum;
     float step_limit = cms * units::water_density / (sp * rho);
     int   br;
@@ -407,7 +433,8 @@ MQI_DI void stepping(ctx& c, trk& t, float rho, float d2b, float& sp) {
     along_step<DBG>(c, t, len, rho, sp);
     if (br != 1) return;
     if (t.ke1 <= 0.5f) return;
-    float u = cs_sum * c.rng.uniform();
+    c.rng.begin_interaction();
+    float u = cs_sum * c.rng.int_uniform(0);
     t.d1    = t.d0;
     if (u < cs0) {
         post_ion<DBG>(c, t);
@@ -488,31 +515,34 @@ MQI_DI void dij_insert(const transport_params& P, uint64_t item, uint32_t voxel,
 }
 
 // Deposits of consecutive steps into the same voxel (and scenario) are
-// summed in registers and added to the scorer arrays once, when the lane
-// moves on (each-scorer-once scoring only; the sums differ from per-step
-// atomics only in FP64 summation order).  Scorer slots beyond MQI_DEFER are
-// scored per step.
-#define MQI_DEFER 4
+// summed per lane and added to the scorer arrays once, when the lane moves
+// on (each-scorer-once scoring only; the sums differ from per-step atomics
+// only in FP64 summation order).  Scorer slots beyond MQI_DEFER are scored
+// per step.  The sums live in the lane's LDS slots (v[s * blockDim.x]), not
+// in registers: held across the loop they pushed the kernel into scratch,
+// and every scratch reload waits on the one counter the atomics in flight
+// also hold.
 struct pending {
     int64_t  cnb;   // voxel of the pending sums, -1: none
     uint64_t off;   // scenario offset of the pending sums
-    double   v[MQI_DEFER];
 };
 
-MQI_DI void flush(const transport_params& P, pending& A) {
+MQI_DI void flush(const transport_params& P, pending& A, double* v) {
     if (A.cnb >= 0) {
         // accumulated slots are positive, +inf or NaN (the per-step path adds
         // every value that is not <= 0, NaN included); untouched slots are 0
 #pragma unroll
-        for (int s = 0; s < MQI_DEFER; ++s)
-            if (A.v[s] != 0.0) atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, A.v[s]);
+        for (int s = 0; s < MQI_DEFER; ++s) {
+            const double x = v[s * blockDim.x];
+            if (x != 0.0) atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, x);
+        }
     }
     A.cnb = -1;
 }
 
 template<bool DBG>
 MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float density, float volume, float sp,
-                  uint64_t dij_item, uint64_t scen_off, pending& A) {
+                  uint64_t dij_item, uint64_t scen_off, pending& A, double* pv) {
     const int     n   = P.n_scorers;
     const int32_t roi = (int32_t) (uint32_t) cnb;   // roi_t DIRECT
     // regular scorers score through roi_->idx(cnb) > 0, the statistics pair
@@ -548,11 +578,11 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
     }
     if (P.loops == MQI_LOOPS_ONCE) {
         if ((int64_t) cnb != A.cnb || scen_off != A.off) {
-            flush(P, A);
+            flush(P, A, pv);
             A.cnb = (int64_t) cnb;
             A.off = scen_off;
 #pragma unroll
-            for (int s = 0; s < MQI_DEFER; ++s) A.v[s] = 0.0;
+            for (int s = 0; s < MQI_DEFER; ++s) pv[s * blockDim.x] = 0.0;
         }
 #pragma unroll
         for (int s = 0; s < MQI_MAX_SCORERS; ++s) {
@@ -563,7 +593,7 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             if (k == MQI_KIND_DIJ)
                 dij_insert(P, dij_item, (uint32_t) cnb, v[s]);
             else if (s < MQI_DEFER)
-                A.v[s] += v[s];
+                pv[s * blockDim.x] += v[s];
             else
                 atomicAdd(P.out[s] + scen_off + cnb, v[s]);
         }
@@ -819,7 +849,7 @@ MQI_DI void trace_record(const transport_params& P, lane& L, uint64_t cnb) {
 // in a beamline child (no scoring there; leaving it hands the track to the
 // next child).  One stepping call site serves both.
 template<bool DBG, bool TRACE>
-MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb) {
+MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb, double* pv) {
     trk&       t  = L.t;
     const bool bl = c.child < c.nbl;
     uint64_t   cnb;
@@ -834,9 +864,17 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb) {
         t.its              = intersect_in(gb, t.p0, t.d0, t.cx, t.cy, t.cz);
         rho                = P.bl_density[off + cnb];
     } else {
-        cnb   = (uint64_t) (int32_t) (t.cz * c.g.nx * c.g.ny + t.cy * c.g.nx + t.cx);
-        t.its = intersect_in(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
-        rho   = P.density[cnb] * L.scale;
+        cnb              = (uint64_t) (int32_t) (t.cz * c.g.nx * c.g.ny + t.cy * c.g.nx + t.cx);
+        const float dens = P.density[cnb];
+        t.its            = intersect_in(c.g, t.p0, t.d0, t.cx, t.cy, t.cz);
+        rho              = dens * L.scale;
+        // The deposits gathered in the voxel the lane has left go out here,
+        // after the wait for the density and ahead of the physics of the
+        // step: one counter tracks loads, stores and atomics alike, so atomics
+        // issued just before the next wait on a load would add their round
+        // trip to every step.
+        if (P.loops == MQI_LOOPS_ONCE && ((int64_t) cnb != L.acc.cnb || L.scen_off != L.acc.off))
+            flush(P, L.acc, pv);
     }
     float sp;   // stopping power ratio of the step
     stepping<DBG>(c, t, rho, t.its, sp);
@@ -877,7 +915,7 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb) {
         float     vol = c.g.X(i + 1) - c.g.X(i);
         vol *= c.g.Y(j + 1) - c.g.Y(j);
         vol *= c.g.Z(k + 1) - c.g.Z(k);
-        score<DBG>(P, t, cnb, rho, vol, sp, L.dij_item, L.scen_off, L.acc);
+        score<DBG>(P, t, cnb, rho, vol, sp, L.dij_item, L.scen_off, L.acc, pv);
     }
     if (!t.stopped) {
         step_axis(t.p1.x, t.d1.x, c.g.xe, c.g.sx, t.cx);
@@ -945,6 +983,9 @@ k_transport(transport_params P) {
     c.nbl   = P.n_bl;
     c.blxf  = P.bl_xf;
     const float* shb = c.g.ze + P.nz + 1;   // beamline edges in LDS
+    // the lane's pending deposit sums (MQI_DEFER doubles, blockDim.x apart),
+    // after the beamline edges at an 8-byte boundary
+    double* pv = reinterpret_cast<double*>(sh + mqi_pend_offset(P.nx, P.ny, P.nz, P.bl_n_edges)) + threadIdx.x;
     const uint64_t tid      = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t nthreads = (uint64_t) gridDim.x * blockDim.x;
     c.stack                 = P.stack + tid * (uint64_t) P.stack_cap * MQI_STACK_FLOATS;
@@ -1067,7 +1108,7 @@ k_transport(transport_params P) {
             } else {
                 sample_vertex(P, P.spots[spot_id], spot_id, scen, L.hist + hoff, L.t);
             }
-            c.rng.s.init(P.seed, (uint32_t) (L.hist + hoff), spot_id, scen, mqi_rng::STREAM_TRANSPORT);
+            c.rng.init(P.seed, (uint32_t) (L.hist + hoff), spot_id, scen, mqi_rng::STREAM_TRANSPORT);
             L.nsteps = 0;
             L.tsteps = 0;
             ++done;
@@ -1086,10 +1127,10 @@ k_transport(transport_params P) {
             L.active  = enter_chain(P, c, L.t, shb);
             if (!L.active) continue;
         }
-        step<DBG, TRACE>(P, c, L, shb);
+        step<DBG, TRACE>(P, c, L, shb, pv);
         ++nstep;
     }
-    flush(P, L.acc);
+    flush(P, L.acc, pv);
     atomicAdd(P.tracked, done);
     atomicAdd(P.steps, nstep);
     if (c.drops) atomicAdd(P.drops, (unsigned long long) c.drops);

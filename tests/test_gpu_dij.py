@@ -83,12 +83,19 @@ def test_dij_threshold(ctx):
         assert cut.indptr[r + 1] - cut.indptr[r] in (keep.sum(), keep.sum() - 1, keep.sum() + 1)
 
 
-def test_dij_products_fp64(ctx):
+@pytest.mark.parametrize("histories", [150, 2])
+def test_dij_products_fp64(ctx, histories):
+    """150 histories per spot: rows of thousands of entries (D^T g over
+    16-bit tile-relative columns); 2: rows of a few hundred (the packed
+    CSR-Adaptive segments with 32-bit columns)."""
     import torch
     g, sp = _setup(ctx)
+    sp["histories"] = np.where(sp["histories"] > 0, histories, 0)
+    ctx.set_spots(sp)
     ctx.run(ctx.config(seed=5, batched=True, dij=True, dij_capacity=1 << 15))
     ctx.dij_build(1e-4)
     D = _csr(ctx)   # spots x voxels (rows = spots)
+    assert (D.nnz >= 1024 * D.shape[0]) == (histories == 150)
     rng = np.random.default_rng(1)
     w = rng.uniform(0.5, 1.5, D.shape[0]).astype(np.float32)
     gv = rng.normal(size=D.shape[1]).astype(np.float32)

@@ -117,3 +117,15 @@ def test_graph_with_stat_rounds(ctx):
         out.append([ctx.scorer(k) for k in range(3)])
     for a, b in zip(*out):
         assert np.allclose(a, b, rtol=1e-9, atol=0)
+
+
+def test_atomic_probe_times_and_rejects_bad_sizes(ctx):
+    """mqi_atomic_probe (the ceiling bench.py's atomic roofline is priced
+    against) runs and reports a positive device time; slot counts beyond
+    2^32 and empty sizes are argument errors, not launches."""
+    ms = ctx.atomic_probe(1 << 20, 1 << 22)
+    assert ms > 0.0
+    assert (1 << 22) / (ms * 1e-3) > 1e8    # at least 1e8 atomics/s
+    for n_slots, n_updates in [((1 << 32) + 1, 1 << 10), (0, 1 << 10), (1 << 10, 0)]:
+        with pytest.raises(RuntimeError):
+            ctx.atomic_probe(n_slots, n_updates)
