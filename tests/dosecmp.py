@@ -63,6 +63,13 @@ def gamma_pass_rate(ref, ev, spacing, dd=0.01, dta=1.0, threshold=0.1, refine=1)
     return float(np.mean(best[mask] <= 1.0)), int(mask.sum())
 
 
+def rebin_xy(a3, f):
+    """Sum of f x f lateral (y, x) voxel blocks of a (z, y, x) array divided
+    by f * f: the dose of the coarser voxel."""
+    nz, ny, nx = a3.shape
+    return a3.reshape(nz, ny // f, f, nx // f, f).sum((2, 4)) / (f * f)
+
+
 def r80(depth_centres, idd):
     """Distal depth where the integral depth dose falls to 80 % of its
     maximum for the last time (linear interpolation between voxel centres).

@@ -27,6 +27,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+import dosecmp  # noqa: E402
 import phantoms  # noqa: E402
 import refbind as rb  # noqa: E402
 
@@ -334,19 +335,12 @@ def _box_rel(d3, s3, margin):
     return np.concatenate([lo, hi]).astype(np.int32), np.array(d3.max()), q, s
 
 
-def rebin_xy(a3, f):
-    """Sum of f x f lateral (y, x) voxel blocks of a (z, y, x) array divided
-    by f * f: the dose of the coarser voxel."""
-    nz, ny, nx = a3.shape
-    return a3.reshape(nz, ny // f, f, nx // f, f).sum((2, 4)) / (f * f)
-
-
 def volumes_fixture(threads, c2_per_spot=312500, c3_per_spot=40000, n_batches=8, only=None):
     """Volumes for 3-D gamma tests.
 
     c2  the slab workload (25 spots 70-230 MeV, release physics), n_batches x
         c2_per_spot histories per spot, dose; the 0.5 x 0.5 x 1 mm voxels
-        rebinned laterally 2 x 2 into 1 x 1 x 1 mm (rebin_xy), kept in the box
+        rebinned laterally 2 x 2 into 1 x 1 x 1 mm (dosecmp.rebin_xy), kept in the box
         above 10 % of the maximum plus 2 voxels.
     c3  the 100-spot sub-plan on the cropped CT (phantoms.ct_crop_plan), n_batches x
         c3_per_spot histories per spot, dose on the 0.98 x 0.98 x 2 mm voxels,
@@ -362,7 +356,7 @@ def volumes_fixture(threads, c2_per_spot=312500, c3_per_spot=40000, n_batches=8,
         w = workloads.slabs()
         g = rb.Grid(w.xe, w.ye, w.ze, w.density)
         runs = _batches(g, [s.copy() for s in w.spots], 0, c2_per_spot, n_batches, rb.SC_DOSE, threads, 6000)
-        vols = [rebin_xy(r[0].reshape(350, 200, 200), 2) for r in runs]
+        vols = [dosecmp.rebin_xy(r[0].reshape(350, 200, 200), 2) for r in runs]
         m, se = _mean_se(vols)
         box, scale, d, s = _box_rel(m, se, 2)
         out.update({"c2_box": box, "c2_dose_scale": scale, "c2_dose_rel": d, "c2_dose_se_rel": s,

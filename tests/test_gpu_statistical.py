@@ -340,3 +340,84 @@ def test_production_fp32_depth_dose_against_fp64(ctx):
     assert abs(d_r80) <= 0.1 and abs(d_r80) <= 3.0 * s_r80 + 0.02
     assert ok[keep].mean() >= 0.9
     assert abs(ratio - 1.0) <= 0.005 and abs(ratio - 1.0) <= 3.0 * s_ratio + 1e-4
+
+
+GOLD_VOL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_volumes.npz")
+
+
+def _has_volume(key):
+    return os.path.exists(GOLD_VOL) and key in np.load(GOLD_VOL).files
+
+
+def _volume(ref, key):
+    """The fixture's box (z0 y0 x0 z1 y1 x1), mean dose and its standard
+    error (Gy per history) from the quantised arrays (make_golden.py
+    _quantise: uint16 steps of 1 / DOSE_Q of the maximum, float16 errors)."""
+    scale = float(ref[key + "_dose_scale"])
+    d = ref[key + "_dose_rel"].astype(np.float64) * (scale / 16384.0)
+    s = ref[key + "_dose_se_rel"].astype(np.float64) * scale
+    return ref[key + "_box"], d, s
+
+
+def _volume_checks(name, rd, rs, gd, gs, spacing, dof):
+    rate, n = dosecmp.gamma_pass_rate(rd, gd, spacing, dd=0.01, dta=1.0, threshold=0.1, refine=2)
+    mask = rd >= 0.1 * rd.max()
+    frac = dosecmp.within_2sigma(gd, gs, rd, rs, mask)
+    want = dosecmp.two_sigma_expectation(gs, rs, dof, dof, mask)
+    ratio = gd.sum() / rd.sum()
+    print("%s 3-D: gamma 1%%/1mm %.4f of %d voxels, within 2 sigma %.4f (equal physics: %.4f), "
+          "box dose ratio %.5f" % (name, rate, n, frac, want, ratio))
+    assert rate >= 0.99
+    assert frac >= want - 0.02
+    assert abs(ratio - 1.0) <= 0.005
+
+
+@pytest.mark.skipif(not _has_volume("c2_box"), reason="reference_volumes.npz has no slab volume")
+def test_slabs_volume_gamma_3d(ctx):
+    """The slab workload (25 spots 70-230 MeV, release physics) in 3-D: the
+    reference's dose over the box above 10 % of its maximum (plus the gamma
+    reach) on the 0.5 x 0.5 x 1 mm voxels rebinned laterally 2 x 2 into
+    1 mm cubes (reference 8 x 312,500 histories per spot, tests/golden/
+    make_golden.py --volumes c2), against the production kernels' 8 x 1.25e6
+    per spot rebinned the same way: 3-D gamma 1 %/1 mm >= 99 %, the 2-sigma
+    fraction and the dose in the box."""
+    from moquimc_amd import workloads
+    ref = np.load(GOLD_VOL)
+    (z0, y0, x0, z1, y1, x1), rd, rs = _volume(ref, "c2")
+    w = workloads.slabs()
+    runs = _gpu_batches(ctx, w, 8, 1250000, 1, seed=24)
+    vols = [dosecmp.rebin_xy(r[0].reshape(350, 200, 200), 2)[z0:z1, y0:y1, x0:x1] for r in runs]
+    del runs
+    gd, gs = _mean_se(vols)
+    _volume_checks("slabs", rd, rs, gd, gs, (1.0, 1.0, 1.0), 7)
+
+
+@pytest.mark.skipif(not _has_volume("c3_box") or not os.path.exists(GOLD_CT),
+                    reason="reference_volumes.npz has no CT sub-plan volume")
+def test_ct_subplan_volume_gamma_3d(ctx):
+    """The 100-spot sub-plan on the cropped CT (reference_ct.npz inputs) in
+    3-D on its 0.98 x 0.98 x 2 mm voxels: the reference's dose over the box
+    above 10 % of its maximum plus the gamma reach (8 x 4e4 histories per
+    spot, make_golden.py --volumes c3) against the production kernels' 8 x
+    1e5 per spot: 3-D gamma 1 %/1 mm >= 99 %, the 2-sigma fraction and the
+    dose in the box."""
+    ct = np.load(GOLD_CT)
+    ref = np.load(GOLD_VOL)
+    (z0, y0, x0, z1, y1, x1), rd, rs = _volume(ref, "c3")
+    xe, ye, ze, hu, spots = ct["c3_xe"], ct["c3_ye"], ct["c3_ze"], ct["c3_hu"], ct["c3_spots"]
+    nz, ny, nx = hu.shape
+    ctx.set_grid_hu(xe, ye, ze, hu)
+    sp = spots.copy()
+    per_spot = 100000
+    sp["histories"] = per_spot
+    ctx.set_spots(sp)
+    ctx.set_scorers(1)
+    vols = []
+    for b in range(8):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=33, batched=True, hist_offset=b * per_spot))
+        assert res.status == 0 and res.histories == per_spot * len(sp)
+        vols.append(ctx.scorer(0).reshape(nz, ny, nx)[z0:z1, y0:y1, x0:x1] / res.histories)
+    gd, gs = _mean_se(vols)
+    spacing = (float(ze[1] - ze[0]), float(ye[1] - ye[0]), float(xe[1] - xe[0]))
+    _volume_checks("CT sub-plan", rd, rs, gd, gs, spacing, 7)
