@@ -31,6 +31,7 @@ def log(*a):
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
 TRAFFIC_SOURCE = "profiles/hbm_counters_ct2000_dij5000.md (rocprofv3 FETCH_SIZE and WRITE_SIZE, copy-calibrated)"
+COUNTER_SOURCE = "profiles/transport_counters_ct2000.md (rocprofv3 SQ_* and TCC_ATOMIC of the same pass)"
 
 
 def measured_traffic(key):
@@ -43,10 +44,15 @@ def measured_traffic(key):
         return None
 
 
-def transport_roofline(steps, kernel_ms, launches, n_scorers):
+def transport_roofline(steps, kernel_ms, launches, n_scorers, atomic_ceiling):
     """Algorithmic bytes of the transport: per voxel step one density load
     (4 B) and one FP64 read-modify-write per scorer (8 B each), over the
-    average duration of one persistent launch (HIP events on its stream)."""
+    average duration of one persistent launch (HIP events on its stream).
+    Beside it the two ceilings the counters point at: the VALU issue share
+    and lane activity of the pass (SQ counters), and its FP64 atomic rate
+    (TCC_ATOMIC per voxel step from the counter run, scaled to this launch)
+    against the rate mqi_atomic_probe measures for random FP64 atomics on
+    this device (atomic_ceiling, atomics/s, None when not measured)."""
     bytes_per_step = 4.0 + 8.0 * n_scorers
     per_launch = steps / launches * bytes_per_step
     ms_per_launch = kernel_ms / launches
@@ -54,14 +60,33 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers):
     m = measured_traffic("k_transport")
     # counters per voxel step, scaled to this launch's steps
     traffic = None if m is None else round(m["traffic_bytes"] / m["voxel_steps"] * steps / launches)
-    return {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-            "traffic_source": TRAFFIC_SOURCE if m else None,
-            "kernel": "k_transport<release, dynamic> (persistent)", "launch_ms": round(ms_per_launch, 3),
-            "voxel_steps_per_launch": int(steps / launches), "bytes_per_step": bytes_per_step,
-            "scorer_deposits_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1),
-            "limiter": "VALU issue (~88% of cycles, SQ counters) and the FP64 atomic stream; neither HBM nor "
-                       "MFMA bounds this pass (profiles/hbm_counters_ct2000_dij5000.md)"}
+    out = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+           "traffic_source": TRAFFIC_SOURCE if m else None,
+           "kernel": "k_transport<release, dynamic> (persistent)", "launch_ms": round(ms_per_launch, 3),
+           "voxel_steps_per_launch": int(steps / launches), "bytes_per_step": bytes_per_step,
+           "scorer_deposits_per_s": round(steps * n_scorers / (kernel_ms * 1e-3), 1)}
+    sq = measured_traffic("k_transport_sq")
+    if sq:
+        out["valu_issue_frac"] = sq["valu_issue_frac"]
+        out["valu_lanes_active"] = sq["valu_lanes_active"]
+    if m and "atomics" in m:
+        rate = m["atomics"] / m["voxel_steps"] * steps / launches / (ms_per_launch * 1e-3)
+        out["atomic_roofline"] = {"achieved": float("%.4g" % rate), "peak": atomic_ceiling, "unit": "atomics/s",
+                                  "frac": round(rate / atomic_ceiling, 4) if atomic_ceiling else None,
+                                  "atomics_per_voxel_step": round(m["atomics"] / m["voxel_steps"], 4),
+                                  "peak_source": "mqi_atomic_probe, random FP64 atomic adds on this device, this run",
+                                  "counter_source": COUNTER_SOURCE}
+    out["limiter"] = ("the FP64 atomic stream of the two scored arrays (atomic_roofline); VALU issue and HBM "
+                      "bandwidth are not at their ceilings (" + COUNTER_SOURCE + ")")
+    return out
+
+
+def atomic_ceiling(ctx, n_slots=2 * 512 * 512 * 200, n_updates=1 << 30):
+    """Random FP64 atomic adds per second on this device (mqi_atomic_probe
+    over two CT-dose-sized arrays)."""
+    ms = ctx.atomic_probe(n_slots, n_updates)
+    return float("%.4g" % (n_updates / (ms * 1e-3)))
 
 
 def dose_agreement(batched, plain, w, dd=0.01, dta=1.0, threshold=0.1):
@@ -181,7 +206,8 @@ def _cpu_baseline(w, cores, target_s):
 
 def adjoint_bench(ctx_cls, reps=20):
     """Dij workload: transport with the Dij scorer, CSR build (1e-4 of spot
-    max), then time D^T g with HIP events."""
+    max), then D^T g and D w timed with HIP events: the median of reps
+    products after reps untimed ones."""
     import ctypes
 
     from moquimc_amd import _abi, workloads
@@ -199,7 +225,7 @@ def adjoint_bench(ctx_cls, reps=20):
     dev = torch.device("cuda", 0)
     g = torch.randn(cols, device=dev, dtype=torch.float32)
     y = torch.zeros(rows, device=dev, dtype=torch.float64)
-    for _ in range(3):
+    for _ in range(reps):   # warm-up: clocks settle over the first tens of products
         ctx.dij_matvec(0, 1, g.data_ptr(), y.data_ptr(), y_double=True)
     ms = []
     for _ in range(reps):
@@ -211,7 +237,7 @@ def adjoint_bench(ctx_cls, reps=20):
     # forward D w on the voxel-major copy (float32 dose out)
     w8 = torch.rand(rows, device=dev, dtype=torch.float32) + 0.5
     d = torch.zeros(cols, device=dev, dtype=torch.float32)
-    for _ in range(3):
+    for _ in range(reps):
         ctx.dij_matvec(0, 0, w8.data_ptr(), d.data_ptr(), y_double=False)
     fms = []
     for _ in range(reps):
@@ -226,10 +252,11 @@ def adjoint_bench(ctx_cls, reps=20):
             "forward_ms": round(f_med, 4), "forward_GBs": round(f_bytes / (f_med * 1e-3) / 1e9, 1),
             "adjoint_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": round(gbs / HBM_PEAK_GBS, 4),
-                                 "traffic": (round(measured_traffic("k_csr_adaptive_adjoint")["traffic_bytes"])
-                                             if measured_traffic("k_csr_adaptive_adjoint") else None),
+                                 "traffic": (round(measured_traffic("k_pieces_adjoint")["traffic_bytes"])
+                                             if measured_traffic("k_pieces_adjoint") else None),
                                  "traffic_source": TRAFFIC_SOURCE,
-                                 "kernel": "k_csr_adaptive + k_csr_split_rows (D^T g)", "algorithmic_bytes": int(alg_bytes)}}
+                                 "kernel": "k_gather_x + k_pieces_staged + k_piece_rows (D^T g)",
+                                 "algorithmic_bytes": int(alg_bytes)}}
 
 
 def phantom_bench(ctx_cls, reps=3):
@@ -263,6 +290,24 @@ def phantom_bench(ctx_cls, reps=3):
                     "kernel_histories_per_s": round(w.histories / (kms / 1e3), 1)}
         log("%s: %.4f s wall, %.0f histories/s" % (key, best, w.histories / best))
     return {"phantoms": out}
+
+
+def transport_pass(waves=3, block=128, fetch=2, refill=8):
+    """One production batched pass of the 2000-spot CT plan (dose + the
+    statistics pair) with a fixed launch shape: a single k_transport dispatch
+    for the counter runs."""
+    from moquimc_amd import _abi, workloads
+    w = workloads.ct()
+    w.mask = _abi.SCORE_DOSE | _abi.SCORE_STAT
+    ctx = _abi.Context(0)
+    w.load(ctx)
+    cfg = ctx.config(seed=11, batched=True)
+    cfg.waves_per_simd, cfg.block_size, cfg.fetch_chunk, cfg.refill_lanes = waves, block, fetch, refill
+    r = ctx.run(cfg)
+    ctx.close()
+    print(json.dumps({"histories": int(r.histories), "voxel_steps": int(r.steps), "kernel_ms": round(r.kernel_ms, 2),
+                      "launch": {"waves_per_simd": waves, "block_size": block, "fetch_chunk": fetch,
+                                 "refill_lanes": refill}}), flush=True)
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -360,6 +405,8 @@ def main():
     ap.add_argument("--phantoms-only", action="store_true", help="only the water and slab phantom leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the reference kernel of the HBM counter runs)")
+    ap.add_argument("--transport-pass", action="store_true",
+                    help="only one production pass of the CT plan with a fixed launch shape (the counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take its own slice "
@@ -368,6 +415,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.transport_pass:
+        transport_pass()
         return
     if args.robust_only:
         from moquimc_amd import _abi
@@ -464,7 +514,7 @@ def main():
            "batched_vs_plain_dose": agree,
            "secondaries_dropped_per_step": round(dropped / args.steps, 1),
            "steps_per_history": round(steps / max(1, hist), 1),
-           "roofline": transport_roofline(steps, kernel_ms, launches, n_scorers)}
+           "roofline": transport_roofline(steps, kernel_ms, launches, n_scorers, atomic_ceiling(ctx))}
     ctx.close()
     if not args.no_adjoint:
         try:

@@ -51,17 +51,18 @@ struct csr_dev {
     int32_t* nzvox = nullptr;
     int32_t* ccol  = nullptr;
     float*   xc    = nullptr;
-    // D^T g over 16-bit deltas of the compacted columns, for matrices whose
-    // rows are long (mqi_aux::pieces_staged); ccol is dropped once they are
-    // built
+    // D^T g over byte deltas (with escapes) of the compacted columns, for
+    // matrices whose rows are long (mqi_aux::pieces_staged); ccol is dropped
+    // once they are built
     int64_t   n_piece   = 0;
-    int32_t*  pbase     = nullptr;   // n_piece: the column the piece's first delta adds to
     int32_t*  row_piece = nullptr;   // rows + 1: each row's first piece
     double*   ppart     = nullptr;   // n_piece: sums of the pieces
-    int64_t*  pb        = nullptr;   // n_piece + 1: staged start of each piece (whole 256-nonzero batches)
     float*    sval      = nullptr;   // staged values (lane-major batches, zero padding)
-    uint16_t* sdcol     = nullptr;   // staged deltas
-    int32_t*  porder    = nullptr;   // n_piece: the order the product processes the pieces in
+    uint8_t*  sd8       = nullptr;   // staged deltas, one byte each (255: escape)
+    uint16_t* pesc      = nullptr;   // the escapes of all pieces (+ 64 padding entries)
+    // n_piece headers in the order the product runs the pieces: first batch,
+    // batches, the column the first delta adds to, piece index
+    mqi_aux::piece_hdr* phdr = nullptr;
     // D w over 16-bit spot columns (at most 65536 spots; tcol is then dropped)
     uint16_t* tcol16 = nullptr;
 };
@@ -262,11 +263,10 @@ free_dij(mqi_ctx* c) {
         dfree(m.nzvox);
         dfree(m.ccol);
         dfree(m.xc);
-        dfree(m.pb);
         dfree(m.sval);
-        dfree(m.sdcol);
-        dfree(m.porder);
-        dfree(m.pbase);
+        dfree(m.sd8);
+        dfree(m.pesc);
+        dfree(m.phdr);
         dfree(m.row_piece);
         dfree(m.ppart);
         dfree(m.tcol16);
@@ -325,11 +325,12 @@ build_plan(mqi_ctx* c, const std::vector<int64_t>& ip, csr_plan& P) {
     return 0;
 }
 
-// The pieces of D^T g over 16-bit column deltas (mqi_aux::pieces_staged):
-// the deltas of the compacted columns, and the positions where a row skips
-// more than 65535 columns, come from the device; each row is cut there and
-// every MQI_PIECE nonzeros (cut_pieces), then the pieces are restaged in
-// whole lane-major batches and ordered by their first columns.
+// The pieces of D^T g over column deltas (mqi_aux::pieces_staged): the
+// 16-bit deltas of the compacted columns, and the positions where a row
+// skips more than 65535 columns, come from the device; each row is cut there
+// and every MQI_PIECE nonzeros (cut_pieces), then the pieces are restaged in
+// whole lane-major batches with one byte per delta (the deltas of 255 and
+// more in per-piece escape lists) and ordered by their first columns.
 int
 build_pieces(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& ip) {
     const int64_t       rows   = m.rows;
@@ -369,25 +370,69 @@ build_pieces(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& ip) {
     m.n_piece = (int64_t) pk.size() - 1;
     std::vector<int64_t> pb(m.n_piece + 1, 0);
     for (int64_t i = 0; i < m.n_piece; ++i) pb[i + 1] = pb[i] + ((pk[i + 1] - pk[i] + 255) & ~(int64_t) 255);
-    if ((r = dalloc(c, &d_pk, pk.size())) || (r = dalloc(c, &m.pbase, std::max<int64_t>(m.n_piece, 1))) ||
-        (r = dalloc(c, &m.row_piece, rows + 1)) || (r = dalloc(c, &m.ppart, std::max<int64_t>(m.n_piece, 1))) ||
-        (r = dalloc(c, &m.pb, pb.size())) || (r = dalloc(c, &m.sval, std::max<int64_t>(pb.back(), 1))) ||
-        (r = dalloc(c, &m.sdcol, std::max<int64_t>(pb.back(), 1))) ||
-        (r = dalloc(c, &m.porder, std::max<int64_t>(m.n_piece, 1))))
+    if ((pb.back() >> 8) > (int64_t) UINT32_MAX) return fail(c, MQI_E_CAPACITY, "too many Dij batches");
+    int32_t*  d_base = nullptr;
+    int64_t*  d_pb   = nullptr;
+    uint32_t* d_ecnt = nullptr;
+    struct guard2 {
+        int32_t*&  b;
+        int64_t*&  p;
+        uint32_t*& e;
+        ~guard2() {
+            hipFree(b);
+            hipFree(p);
+            hipFree(e);
+        }
+    } g2{ d_base, d_pb, d_ecnt };
+    const int64_t np1 = std::max<int64_t>(m.n_piece, 1);
+    if ((r = dalloc(c, &d_pk, pk.size())) || (r = dalloc(c, &d_base, np1)) || (r = dalloc(c, &d_ecnt, np1)) ||
+        (r = dalloc(c, &m.row_piece, rows + 1)) || (r = dalloc(c, &m.ppart, np1)) ||
+        (r = dalloc(c, &d_pb, pb.size())) || (r = dalloc(c, &m.sval, std::max<int64_t>(pb.back(), 1))) ||
+        (r = dalloc(c, &m.sd8, std::max<int64_t>(pb.back(), 1))) || (r = dalloc(c, &m.phdr, np1)))
         return r;
     HIPCHK(c, hipMemcpy(d_pk, pk.data(), sizeof(int64_t) * pk.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(m.row_piece, row_piece.data(), sizeof(int32_t) * (rows + 1), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(m.pb, pb.data(), sizeof(int64_t) * pb.size(), hipMemcpyHostToDevice));
-    KCHK(c, mqi_aux::piece_base(m.ccol, d_dcol, d_pk, m.n_piece, m.pbase, c->stream));
-    KCHK(c, mqi_aux::stage_pieces(m.val, d_dcol, d_pk, m.pb, m.n_piece, m.sval, m.sdcol, c->stream));
+    HIPCHK(c, hipMemcpy(d_pb, pb.data(), sizeof(int64_t) * pb.size(), hipMemcpyHostToDevice));
+    KCHK(c, mqi_aux::piece_base(m.ccol, d_dcol, d_pk, m.n_piece, d_base, c->stream));
+    // escapes (deltas of 255 and more): counted per piece, their lists placed
+    // by the exclusive prefix sum, padded by the 64 entries the product may
+    // read past the last one
+    std::vector<uint32_t> ecnt(m.n_piece), ebase(np1, 0);
+    if (m.n_piece) {
+        KCHK(c, mqi_aux::piece_escapes(d_dcol, d_pk, m.n_piece, d_ecnt, c->stream));
+        HIPCHK(c, hipMemcpy(ecnt.data(), d_ecnt, sizeof(uint32_t) * m.n_piece, hipMemcpyDeviceToHost));
+    }
+    uint64_t n_esc = 0;
+    for (int64_t i = 0; i < m.n_piece; ++i) {
+        ebase[i] = (uint32_t) n_esc;
+        n_esc += ecnt[i];
+    }
+    if (n_esc + 64 > (uint64_t) UINT32_MAX) return fail(c, MQI_E_CAPACITY, "too many Dij escapes");
+    const uint64_t n_esc_alloc = (n_esc + 64 + 1) & ~(uint64_t) 1;
+    if ((r = dalloc(c, &m.pesc, n_esc_alloc))) return r;
+    KCHK(c, mqi_aux::fill_u32(m.pesc, 0u, n_esc_alloc / 2, c->stream));
+    if (m.n_piece) HIPCHK(c, hipMemcpy(d_ecnt, ebase.data(), sizeof(uint32_t) * m.n_piece, hipMemcpyHostToDevice));
+    KCHK(c, mqi_aux::stage_pieces(m.val, d_dcol, d_pk, d_pb, d_ecnt, m.n_piece, m.sval, m.sd8, m.pesc, c->stream));
     // processed in the order of their first columns: the pieces that run at
     // the same time gather from the same stretch of x (7 % faster on the
-    // 5000-spot Dij than row order)
+    // 5000-spot Dij than row order); one 16-byte header per piece in that
+    // order, so a wave starts a piece after one load
     std::vector<int32_t> first(m.n_piece), order(m.n_piece);
-    if (m.n_piece) HIPCHK(c, hipMemcpy(first.data(), m.pbase, sizeof(int32_t) * m.n_piece, hipMemcpyDeviceToHost));
+    if (m.n_piece) HIPCHK(c, hipMemcpy(first.data(), d_base, sizeof(int32_t) * m.n_piece, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < m.n_piece; ++i) order[i] = (int32_t) i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return first[a] < first[b]; });
-    if (m.n_piece) HIPCHK(c, hipMemcpy(m.porder, order.data(), sizeof(int32_t) * m.n_piece, hipMemcpyHostToDevice));
+    std::vector<mqi_aux::piece_hdr> hdr(m.n_piece);
+    for (int64_t i = 0; i < m.n_piece; ++i) {
+        const int32_t q = order[i];
+        hdr[i].batch0   = (uint32_t) (pb[q] >> 8);
+        hdr[i].batches  = (uint32_t) ((pb[q + 1] - pb[q]) >> 8);
+        hdr[i].base     = first[q];
+        hdr[i].piece    = q;
+        hdr[i].esc0     = ebase[q];
+        hdr[i].pad[0] = hdr[i].pad[1] = hdr[i].pad[2] = 0;
+    }
+    if (m.n_piece)
+        HIPCHK(c, hipMemcpy(m.phdr, hdr.data(), sizeof(mqi_aux::piece_hdr) * m.n_piece, hipMemcpyHostToDevice));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1817,10 +1862,10 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
     if (transpose) {
         // D^T g: rows of the spot-major matrix
         const csr_plan& p = m.plan;
-        if (m.sval) {   // pieces over 16-bit column deltas of the compacted x
+        if (m.sval) {   // pieces over byte column deltas of the compacted x
             KCHK(c, mqi_aux::gather_x(x, m.nzvox, m.n_nz, m.xc, st));
-            KCHK(c, mqi_aux::pieces_staged(m.sval, m.sdcol, m.xc, m.pb, m.pbase, m.porder, m.n_piece, m.row_piece,
-                                           m.rows, m.ppart, y, y_double, alpha, accumulate, st));
+            KCHK(c, mqi_aux::pieces_staged(m.sval, m.sd8, m.pesc, m.xc, m.phdr, m.n_piece, m.row_piece, m.rows,
+                                           m.ppart, y, y_double, alpha, accumulate, st));
         } else {
             const int32_t* colp = m.col;
             const float*   xp   = x;

@@ -42,12 +42,14 @@ int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, co
 int csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
                    const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
                    void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
-// D^T g over 16-bit column deltas (mqi_aux.hip): the deltas of each row's
+// D^T g over column deltas (mqi_aux.hip): the 16-bit deltas of each row's
 // columns (0 at row starts and where a row skips more than 65535 columns,
 // those positions listed in gaps), the column before each piece's first
 // delta, the pieces restaged lane-major in whole 256-nonzero batches (pb:
-// staged starts, n_p + 1 entries), and the product: one wave per piece into
-// part[piece] in the order order[] gives, then each row's pieces summed in
+// staged starts, n_p + 1 entries) with one byte per delta, deltas of 255
+// and more kept in per-piece escape lists (ecount: escapes per piece; ebase:
+// their exclusive prefix sums), and the product: one wave per piece into
+// part[piece] in the order of the headers, then each row's pieces summed in
 // a fixed order.
 #define MQI_PIECE 4096          // nonzeros per piece
 #define MQI_PIECE_WAVES 8       // waves per workgroup of the product
@@ -56,11 +58,21 @@ int deltas16(const int64_t* indptr, const int32_t* col, int64_t rows, int64_t nn
              unsigned long long* n_gaps, int64_t* gaps, int64_t cap, hipStream_t s);
 int piece_base(const int32_t* col, const uint16_t* dcol, const int64_t* pk, int64_t n_p, int32_t* pbase,
                hipStream_t s);
-int stage_pieces(const float* val, const uint16_t* dcol, const int64_t* pk, const int64_t* pb, int64_t n_p,
-                 float* sval, uint16_t* sdcol, hipStream_t s);
-int pieces_staged(const float* sval, const uint16_t* sdcol, const float* x, const int64_t* pb, const int32_t* pbase,
-                  const int32_t* order, int64_t n_p, const int32_t* row_piece, int64_t rows, double* part, void* y,
-                  int y_is_double, double alpha, int accumulate, hipStream_t s);
+int piece_escapes(const uint16_t* dcol, const int64_t* pk, int64_t n_p, uint32_t* ecount, hipStream_t s);
+// esc: the escapes, plus 64 entries of padding the product may read
+int stage_pieces(const float* val, const uint16_t* dcol, const int64_t* pk, const int64_t* pb, const uint32_t* ebase,
+                 int64_t n_p, float* sval, uint8_t* sd8, uint16_t* esc, hipStream_t s);
+// a piece as the product reads it: batches [batch0, batch0 + batches) of
+// the staged arrays, the column its first delta adds to, its index, and the
+// position of its first escape
+struct alignas(16) piece_hdr {
+    uint32_t batch0, batches;
+    int32_t  base, piece;
+    uint32_t esc0, pad[3];
+};
+int pieces_staged(const float* sval, const uint8_t* sd8, const uint16_t* esc, const float* x, const piece_hdr* hdr,
+                  int64_t n_p, const int32_t* row_piece, int64_t rows, double* part, void* y, int y_is_double,
+                  double alpha, int accumulate, hipStream_t s);
 // low 16 bits of int32 columns (D w over at most 65536 spots)
 int low16(const int32_t* col, int64_t nnz, uint16_t* out, hipStream_t s);
 // D^T g on compacted x: xc = x[nzvox]; ccol = rank[col]

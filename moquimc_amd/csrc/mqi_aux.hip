@@ -334,6 +334,7 @@ segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, i
 typedef float    f32x4 __attribute__((ext_vector_type(4)));
 typedef int32_t  i32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // four columns in one load: int32 columns, or 16-bit ones (D w over at most
 // 65536 spots)
@@ -491,20 +492,24 @@ csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, con
                           alpha, accumulate, s);
 }
 
-// ------------------------------------------------------------------ D^T g over 16-bit column deltas
+// ------------------------------------------------------------------ D^T g over column deltas
 // Index compression of the CSR stream (delta coding of the column indices,
 // after K. Kourtis, G. Goumas and N. Koziris, "Optimizing sparse
 // matrix-vector multiplication using index and value compression", CF'08):
 // a nonzero keeps its value and the difference of its column to the
-// previous nonzero of its row in 16 bits, 6 bytes instead of 8.  The
-// nonzeros are cut into pieces of <= MQI_PIECE entries of one row, also
-// where a row skips more than 65535 columns; a piece keeps the column its
-// first delta adds to (pbase).  For the product the pieces are restaged:
-// padded with zero values and zero deltas to whole batches of 256 nonzeros,
-// each batch lane-major (the 16-byte load of lane l holds the batch's
-// nonzeros l, l + 64, l + 128 and l + 192), so that each of the four gathers
-// of a batch reads the columns of 64 consecutive nonzeros: few distinct
-// cache lines per gather instruction instead of every fourth column of 256.
+// previous nonzero of its row.  The nonzeros are cut into pieces of <=
+// MQI_PIECE entries of one row, also where a row skips more than 65535
+// columns; a piece keeps the column its first delta adds to (pbase).  For
+// the product the pieces are restaged: padded with zero values and zero
+// deltas to whole batches of 256 nonzeros, each batch lane-major (the
+// 16-byte value load of lane l holds the batch's nonzeros l, l + 64,
+// l + 128 and l + 192, its 4-byte delta load their deltas), so that each of
+// the four gathers of a batch reads the columns of 64 consecutive nonzeros:
+// few distinct cache lines per gather instruction instead of every fourth
+// column of 256.  A delta takes one byte; the 1.2 % of deltas of 255 and
+// more (the steps between the rows and slices of a spot's footprint on the
+// 5000-spot Dij) are escapes, kept in order in the piece's escape list:
+// 5.02 bytes per nonzero instead of 8.
 // One wave per piece rebuilds the columns with wave prefix sums of the
 // deltas and forms the FP64 sum of the piece into part[piece]; a second pass
 // adds each row's pieces in a fixed order, so results do not depend on
@@ -542,17 +547,51 @@ __global__ void k_piece_base(const int32_t* __restrict__ col, const uint16_t* __
     if (p < n_p) pbase[p] = pk[p] < pk[p + 1] ? col[pk[p]] - (int32_t) dcol[pk[p]] : 0;
 }
 
-// pb[p]: the piece's first staged slot (a multiple of 256); pb[n_p]: total
+// deltas of 255 and more are escapes: the byte holds 255 and the piece's
+// escape list (in nonzero order) the delta
+#define MQI_ESC 255
+
+// ecount[p]: the escapes of piece p; one workgroup of 256 per piece
+__global__ void k_piece_escapes(const uint16_t* __restrict__ dcol, const int64_t* __restrict__ pk, int64_t n_p,
+                                uint32_t* __restrict__ ecount) {
+    typedef hipcub::BlockReduce<uint32_t, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    for (int64_t p = blockIdx.x; p < n_p; p += gridDim.x) {
+        const int64_t kb = pk[p], len = pk[p + 1] - kb;
+        uint32_t      n  = 0;
+        for (int64_t i = threadIdx.x; i < len; i += 256) n += dcol[kb + i] >= MQI_ESC ? 1u : 0u;
+        const uint32_t t = BR(tmp).Sum(n);
+        if (threadIdx.x == 0) ecount[p] = t;
+        __syncthreads();
+    }
+}
+
+// pb[p]: the piece's first staged slot (a multiple of 256); pb[n_p]: total.
+// Nonzero i of a piece goes to batch i / 256, lane i % 64, byte (i / 64) % 4
+// of the lane's word; its escape (if any) to esc[ebase[p] + the escapes
+// before it in the piece].  One workgroup of 256 per piece.
 __global__ void k_stage_pieces(const float* __restrict__ val, const uint16_t* __restrict__ dcol,
-                               const int64_t* __restrict__ pk, const int64_t* __restrict__ pb, int64_t n_p,
-                               float* __restrict__ sval, uint16_t* __restrict__ sdcol) {
+                               const int64_t* __restrict__ pk, const int64_t* __restrict__ pb,
+                               const uint32_t* __restrict__ ebase, int64_t n_p, float* __restrict__ sval,
+                               uint8_t* __restrict__ sd8, uint16_t* __restrict__ esc) {
+    typedef hipcub::BlockScan<uint32_t, 256> BS;
+    __shared__ typename BS::TempStorage tmp;
     for (int64_t p = blockIdx.x; p < n_p; p += gridDim.x) {
         const int64_t kb = pk[p], len = pk[p + 1] - kb, base = pb[p], padded = pb[p + 1] - base;
-        for (int64_t i = threadIdx.x; i < padded; i += blockDim.x) {
-            const int64_t j   = i & 255;
-            const int64_t dst = base + (i & ~(int64_t) 255) + ((j & 63) << 2) + (j >> 6);
-            sval[dst]         = i < len ? val[kb + i] : 0.0f;
-            sdcol[dst]        = i < len ? dcol[kb + i] : (uint16_t) 0;
+        uint32_t      e  = ebase[p];
+        for (int64_t i0 = 0; i0 < padded; i0 += 256) {   // padded is a multiple of 256
+            const int64_t  i   = i0 + threadIdx.x;
+            const int64_t  j   = i & 255;
+            const int64_t  dst = base + i0 + ((j & 63) << 2) + (j >> 6);
+            const uint16_t d   = i < len ? dcol[kb + i] : (uint16_t) 0;
+            const uint32_t f   = d >= MQI_ESC ? 1u : 0u;
+            uint32_t       off, tot;
+            BS(tmp).ExclusiveSum(f, off, tot);
+            sval[dst] = i < len ? val[kb + i] : 0.0f;
+            sd8[dst]  = (uint8_t) (f ? MQI_ESC : d);
+            if (f) esc[e + off] = d;
+            e += tot;
+            __syncthreads();
         }
     }
 }
@@ -570,26 +609,34 @@ MQI_DI uint32_t wave_scan_dpp(uint32_t x) {
 }
 
 // One wave per piece, MQI_PIECE_BATCHES batches (2048 nonzeros) in flight;
-// the pieces run in the order order[] gives (their first columns ascending:
+// the pieces run in the order of their headers (first columns ascending:
 // pieces that run at the same time gather from the same stretch of x).
+// The wave keeps the next 64 escapes of its piece in a register (one per
+// lane) and hands them to the lanes whose byte is an escape.
 __global__ void __launch_bounds__(64 * MQI_PIECE_WAVES)
-    k_pieces_staged(const float* __restrict__ sval, const uint16_t* __restrict__ sdcol, const float* __restrict__ x,
-                    const int64_t* __restrict__ pb, const int32_t* __restrict__ pbase,
-                    const int32_t* __restrict__ order, int64_t n_p, double* __restrict__ part) {
+    k_pieces_staged(const float* __restrict__ sval, const uint8_t* __restrict__ sd8, const uint16_t* __restrict__ esc,
+                    const float* __restrict__ x, const piece_hdr* __restrict__ hdr, int64_t n_p,
+                    double* __restrict__ part) {
     constexpr int U    = MQI_PIECE_BATCHES;
     const int     lane = threadIdx.x & 63;
     const int64_t w    = (int64_t) blockIdx.x * MQI_PIECE_WAVES + (threadIdx.x >> 6);
     const int64_t nw   = (int64_t) gridDim.x * MQI_PIECE_WAVES;
     for (int64_t i = w; i < n_p; i += nw) {
-        const int64_t p  = order[i];
-        const int64_t b0 = pb[p] >> 8, nb = (pb[p + 1] >> 8) - b0;
-        const f32x4*  v4 = reinterpret_cast<const f32x4*>(sval) + b0 * 64;
-        const u16x4*  d4 = reinterpret_cast<const u16x4*>(sdcol) + b0 * 64;
-        uint32_t      run = (uint32_t) pbase[p];
-        double        acc = 0.0;
+        // the whole header in two 16-byte loads
+        const u32x4     h  = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(hdr + i));
+        const u32x4     h2 = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(hdr + i) + 1);
+        const int64_t   p  = (int32_t) h.w;
+        const int64_t   b0 = h.x, nb = h.y;
+        const f32x4*    v4 = reinterpret_cast<const f32x4*>(sval) + b0 * 64;
+        const uint32_t* d4 = reinterpret_cast<const uint32_t*>(sd8) + b0 * 64;
+        uint32_t        run = h.z;
+        // escapes: esc[e_ptr + lane] in e_reg, e_off of them used
+        uint32_t        e_ptr = h2.x, e_off = 0;
+        uint32_t        e_reg = esc[e_ptr + lane];   // the list is padded by 64
+        double          acc   = 0.0;
         for (int64_t b = 0; b < nb; b += U) {
-            f32x4 v[U];
-            u16x4 d[U];
+            f32x4    v[U];
+            uint32_t d[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (b + u < nb) {
@@ -597,20 +644,41 @@ __global__ void __launch_bounds__(64 * MQI_PIECE_WAVES)
                     d[u] = __builtin_nontemporal_load(d4 + (b + u) * 64 + lane);
                 } else {
                     v[u] = f32x4{ 0.0f, 0.0f, 0.0f, 0.0f };
-                    d[u] = u16x4{ 0, 0, 0, 0 };
+                    d[u] = 0u;
                 }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 // quarter q of the batch (nonzeros 64 q .. 64 q + 63, one per
-                // lane): its columns are a prefix sum over the wave
-                const uint32_t s0 = wave_scan_dpp(d[u].x);
+                // lane): byte q of the lane's word, its columns a prefix sum
+                // over the wave
+                uint32_t c[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t                 dq = (d[u] >> (8 * q)) & 0xffu;
+                    const unsigned long long m  = __ballot(dq == MQI_ESC);
+                    if (m) {
+                        const uint32_t n = (uint32_t) __popcll(m);
+                        if (e_off + n > 64) {
+                            e_ptr += e_off;
+                            e_off = 0;
+                            e_reg = esc[e_ptr + lane];
+                        }
+                        const uint32_t k  = e_off + (uint32_t) __builtin_amdgcn_mbcnt_hi(
+                                                       (uint32_t) (m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) m, 0u));
+                        const uint32_t ev = (uint32_t) __shfl((int) e_reg, (int) (k & 63u));
+                        if (dq == MQI_ESC) dq = ev;
+                        e_off += n;
+                    }
+                    c[q] = dq;
+                }
+                const uint32_t s0 = wave_scan_dpp(c[0]);
                 const uint32_t t0 = (uint32_t) __builtin_amdgcn_readlane((int) s0, 63);
-                const uint32_t s1 = wave_scan_dpp(d[u].y);
+                const uint32_t s1 = wave_scan_dpp(c[1]);
                 const uint32_t t1 = (uint32_t) __builtin_amdgcn_readlane((int) s1, 63);
-                const uint32_t s2 = wave_scan_dpp(d[u].z);
+                const uint32_t s2 = wave_scan_dpp(c[2]);
                 const uint32_t t2 = (uint32_t) __builtin_amdgcn_readlane((int) s2, 63);
-                const uint32_t s3 = wave_scan_dpp(d[u].w);
+                const uint32_t s3 = wave_scan_dpp(c[3]);
                 const uint32_t t3 = (uint32_t) __builtin_amdgcn_readlane((int) s3, 63);
                 acc += (double) v[u].x * (double) x[run + s0];
                 acc += (double) v[u].y * (double) x[run + t0 + s1];
@@ -663,22 +731,30 @@ piece_base(const int32_t* col, const uint16_t* dcol, const int64_t* pk, int64_t 
 }
 
 int
-stage_pieces(const float* val, const uint16_t* dcol, const int64_t* pk, const int64_t* pb, int64_t n_p, float* sval,
-             uint16_t* sdcol, hipStream_t s) {
+piece_escapes(const uint16_t* dcol, const int64_t* pk, int64_t n_p, uint32_t* ecount, hipStream_t s) {
     if (n_p <= 0) return 0;
-    hipLaunchKernelGGL(k_stage_pieces, dim3((unsigned) (n_p < 65536 ? n_p : 65536)), dim3(256), 0, s, val, dcol, pk, pb,
-                       n_p, sval, sdcol);
+    hipLaunchKernelGGL(k_piece_escapes, dim3((unsigned) (n_p < 65536 ? n_p : 65536)), dim3(256), 0, s, dcol, pk, n_p,
+                       ecount);
     return (int) hipGetLastError();
 }
 
 int
-pieces_staged(const float* sval, const uint16_t* sdcol, const float* x, const int64_t* pb, const int32_t* pbase,
-              const int32_t* order, int64_t n_p, const int32_t* row_piece, int64_t rows, double* part, void* y,
-              int y_is_double, double alpha, int accumulate, hipStream_t s) {
+stage_pieces(const float* val, const uint16_t* dcol, const int64_t* pk, const int64_t* pb, const uint32_t* ebase,
+             int64_t n_p, float* sval, uint8_t* sd8, uint16_t* esc, hipStream_t s) {
+    if (n_p <= 0) return 0;
+    hipLaunchKernelGGL(k_stage_pieces, dim3((unsigned) (n_p < 65536 ? n_p : 65536)), dim3(256), 0, s, val, dcol, pk, pb,
+                       ebase, n_p, sval, sd8, esc);
+    return (int) hipGetLastError();
+}
+
+int
+pieces_staged(const float* sval, const uint8_t* sd8, const uint16_t* esc, const float* x, const piece_hdr* hdr,
+              int64_t n_p, const int32_t* row_piece, int64_t rows, double* part, void* y, int y_is_double,
+              double alpha, int accumulate, hipStream_t s) {
     if (n_p > 0) {
         const int64_t b = (n_p + MQI_PIECE_WAVES - 1) / MQI_PIECE_WAVES;
         hipLaunchKernelGGL(k_pieces_staged, dim3((unsigned) (b < 65536 ? b : 65536)), dim3(64 * MQI_PIECE_WAVES), 0,
-                           s, sval, sdcol, x, pb, pbase, order, n_p, part);
+                           s, sval, sd8, esc, x, hdr, n_p, part);
     }
     if (rows > 0) {
         const unsigned b = (unsigned) ((rows + 3) / 4);
