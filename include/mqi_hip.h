@@ -181,6 +181,16 @@ int mqi_stat(mqi_ctx* ctx, int scenario, uint64_t n_histories, float threshold, 
  * float32 values and int32 indices, keeping entries >= rel_threshold times
  * the row maximum; columns sorted; a voxel-major copy is kept for D w. */
 int mqi_dij_build(mqi_ctx* ctx, double rel_threshold);
+/* A caller's CSR as scenario `scenario`'s Dij (host arrays: rows + 1 row
+ * pointers from 0, nondecreasing; each row's columns in [0, cols), strictly
+ * ascending; float32 values), prepared for mqi_dij_matvec like a built one
+ * and replacing any Dij of that scenario.  No counterpart in the reference,
+ * which only writes its Dij (io::save_to_npz, mqi_io.hpp:249-320): this
+ * takes such a matrix back (read by the caller, values as float32) so the
+ * products and their gradients run on it.  MQI_E_ARG names the first
+ * offending row or entry. */
+int mqi_dij_set_csr(mqi_ctx* ctx, int scenario, int64_t rows, int64_t cols, const int64_t* indptr,
+                    const int32_t* indices, const float* values);
 int mqi_dij_info(mqi_ctx* ctx, int scenario, int64_t* nnz, int64_t* rows, int64_t* cols);
 int mqi_dij_copy(mqi_ctx* ctx, int scenario, int64_t* indptr, int32_t* indices, float* values);
 int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, const int32_t** indices,

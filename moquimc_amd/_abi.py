@@ -83,6 +83,7 @@ def lib():
             "mqi_dij_info": (i, [vp, i, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                  ctypes.POINTER(ctypes.c_int64)]),
             "mqi_dij_copy": (i, [vp, i, vp, vp, vp]),
+            "mqi_dij_set_csr": (i, [vp, i, ctypes.c_int64, ctypes.c_int64, vp, vp, vp]),
             "mqi_dij_device_ptrs": (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
             "mqi_dij_matvec": (i, [vp, i, i, vp, vp, i, d, i, vp]),
             "mqi_last_matvec_ms": (i, [vp, ctypes.POINTER(d)]),
@@ -107,7 +108,7 @@ def lib():
 EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "mqi_set_grid", "mqi_set_grid_hu",
             "mqi_get_density", "mqi_set_spots", "mqi_set_scenarios", "mqi_set_scorers", "mqi_zero_scorers",
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
-            "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy",
+            "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy", "mqi_dij_set_csr",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
             "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha",
             "mqi_physics_probe", "mqi_run_vertices_noalloc", "mqi_atomic_probe"]
@@ -351,6 +352,31 @@ class Context:
         self._chk(self._L.mqi_dij_info(self.h, scenario, ctypes.byref(nnz), ctypes.byref(rows), ctypes.byref(cols)),
                   "mqi_dij_info")
         return nnz.value, rows.value, cols.value
+
+    def dij_set_csr(self, indptr, indices, values, shape, scenario=0):
+        """A CSR (rows x cols; columns ascending within rows) as the Dij of
+        scenario (mqi_dij_set_csr); values are taken as float32."""
+        rows, cols = int(shape[0]), int(shape[1])
+        indptr = np.ascontiguousarray(indptr, np.int64)
+        indices = np.ascontiguousarray(indices, np.int32)
+        values = np.ascontiguousarray(values, np.float32)
+        if indptr.size != rows + 1 or indices.size < indptr[-1] or values.size < indptr[-1]:
+            raise ValueError("indptr needs rows + 1 entries and indices/values indptr[-1]")
+        self._chk(self._L.mqi_dij_set_csr(self.h, scenario, rows, cols, _p(indptr), _p(indices), _p(values)),
+                  "mqi_dij_set_csr")
+
+    def dij_load_npz(self, path, scenario=0):
+        """A scipy-style CSR .npz (the layout io::save_to_npz and
+        mqi_dij_save_npz write: rows = spots, columns = voxels) as the Dij of
+        scenario; values as float32."""
+        with np.load(path, allow_pickle=False) as z:
+            fmt = z["format"][()]
+            fmt = fmt.decode() if isinstance(fmt, bytes) else str(fmt)
+            if fmt != "csr":
+                raise ValueError("%s is not a CSR matrix (format %r)" % (path, fmt))
+            shape = [int(v) for v in z["shape"]]
+            self.dij_set_csr(z["indptr"].astype(np.int64), z["indices"].astype(np.int32),
+                             z["data"].astype(np.float32), shape, scenario)
 
     def dij_csr(self, scenario=0):
         nnz, rows, cols = self.dij_info(scenario)

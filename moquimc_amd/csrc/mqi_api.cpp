@@ -252,8 +252,8 @@ dfree(T*& p) {
 }
 
 void
-free_dij(mqi_ctx* c) {
-    for (auto& m : c->dij) {
+free_csr(csr_dev& m) {
+    {
         dfree(m.indptr);
         dfree(m.col);
         dfree(m.val);
@@ -277,6 +277,12 @@ free_dij(mqi_ctx* c) {
             dfree(p->part);
         }
     }
+    m = csr_dev();
+}
+
+void
+free_dij(mqi_ctx* c) {
+    for (auto& m : c->dij) free_csr(m);
     c->dij.clear();
 }
 
@@ -1546,6 +1552,99 @@ mqi_stat(mqi_ctx* c, int scenario, uint64_t n, float thr, double* out) {
     return stat_value(c, scenario, n, thr, out);
 }
 
+// The product structures of a CSR on the device (m.indptr, m.col, m.val set;
+// indptr its row pointers on the host): rows sorted by column (unless they
+// are already), the voxel-major copy and plan of D w, the compacted columns
+// of D^T g and, for long rows, its pieces.
+int
+prepare_products(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& indptr, bool sort_rows) {
+    int r = 0;
+    if ((r = build_plan(c, indptr, m.plan))) return r;
+    if (m.nnz > 0) {
+        // sort every row by voxel (rows handed in by the caller are checked
+        // sorted instead)
+        if (sort_rows) {
+            int32_t* tk = nullptr;
+            float*   tv = nullptr;
+            if ((r = dalloc(c, &tk, m.nnz)) || (r = dalloc(c, &tv, m.nnz))) return r;
+            size_t bytes = 0;
+            KCHK(c, mqi_aux::segmented_sort(m.col, m.val, tk, tv, m.nnz, (int) m.rows, m.indptr, m.indptr + 1,
+                                            nullptr, &bytes, c->stream));
+            char* scratch = nullptr;
+            if ((r = dalloc(c, &scratch, std::max<size_t>(bytes, 1)))) return r;
+            KCHK(c, mqi_aux::segmented_sort(m.col, m.val, tk, tv, m.nnz, (int) m.rows, m.indptr, m.indptr + 1,
+                                            scratch, &bytes, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            hipFree(scratch);
+            hipFree(tk);
+            hipFree(tv);
+        }
+        // voxel-major copy for D w: row lengths by counting, entries by a
+        // stable sort on the voxel (each voxel's spots ascending)
+        uint32_t* d_cc = nullptr;
+        if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &m.tindptr, m.cols + 1)) ||
+            (r = dalloc(c, &m.tcol, m.nnz)) || (r = dalloc(c, &m.tval, m.nnz)))
+            return r;
+        KCHK(c, mqi_aux::fill_u32(d_cc, 0u, m.cols, c->stream));
+        KCHK(c, mqi_aux::csr_transpose_count(m.col, m.nnz, d_cc, c->stream));
+        std::vector<uint32_t> cc(m.cols);
+        HIPCHK(c, hipMemcpy(cc.data(), d_cc, sizeof(uint32_t) * m.cols, hipMemcpyDeviceToHost));
+        std::vector<int64_t> tptr(m.cols + 1, 0);
+        for (int64_t v = 0; v < m.cols; ++v) tptr[v + 1] = tptr[v] + cc[v];
+        HIPCHK(c, hipMemcpy(m.tindptr, tptr.data(), sizeof(int64_t) * (m.cols + 1), hipMemcpyHostToDevice));
+        if ((r = build_plan(c, tptr, m.tplan))) return r;
+        // compacted gather of D^T g: voxels with a nonzero, and each one's rank
+        std::vector<int32_t> nz, rank(m.cols, 0);
+        for (int64_t v = 0; v < m.cols; ++v)
+            if (cc[v]) {
+                rank[v] = (int32_t) nz.size();
+                nz.push_back((int32_t) v);
+            }
+        m.n_nz       = (int64_t) nz.size();
+        int32_t* d_rank = nullptr;
+        if ((r = dalloc(c, &m.nzvox, m.n_nz)) || (r = dalloc(c, &m.ccol, m.nnz)) || (r = dalloc(c, &m.xc, m.n_nz)) ||
+            (r = dalloc(c, &d_rank, m.cols)))
+            return r;
+        HIPCHK(c, hipMemcpy(m.nzvox, nz.data(), sizeof(int32_t) * m.n_nz, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d_rank, rank.data(), sizeof(int32_t) * m.cols, hipMemcpyHostToDevice));
+        KCHK(c, mqi_aux::renumber(m.col, m.nnz, d_rank, m.ccol, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        hipFree(d_rank);
+        // long rows (the Dij of pencil beams): D^T g over 16-bit columns
+        if (m.nnz >= (int64_t) MQI_SEG_SHORT * m.rows) {
+            if ((r = build_pieces(c, m, indptr))) return r;
+            dfree(m.ccol);
+        }
+        int key_bits = 1;
+        while (key_bits < 32 && (1ll << key_bits) < m.cols) ++key_bits;
+        int32_t*  d_keys = nullptr;
+        uint64_t *d_pk = nullptr, *d_pk2 = nullptr;
+        char*     d_tmp = nullptr;
+        size_t    tbytes = 0;
+        KCHK(c, mqi_aux::csr_transpose_sorted(m.indptr, m.col, m.val, m.rows, m.nnz, key_bits, nullptr, nullptr,
+                                              nullptr, nullptr, nullptr, nullptr, &tbytes, c->stream));
+        if ((r = dalloc(c, &d_keys, m.nnz)) || (r = dalloc(c, &d_pk, m.nnz)) || (r = dalloc(c, &d_pk2, m.nnz)) ||
+            (r = dalloc(c, &d_tmp, std::max<size_t>(tbytes, 1))))
+            return r;
+        KCHK(c, mqi_aux::csr_transpose_sorted(m.indptr, m.col, m.val, m.rows, m.nnz, key_bits, d_keys, d_pk, d_pk2,
+                                              m.tcol, m.tval, d_tmp, &tbytes, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        hipFree(d_keys);
+        hipFree(d_pk);
+        hipFree(d_pk2);
+        hipFree(d_tmp);
+        hipFree(d_cc);
+        // D w over 16-bit spot columns
+        if (m.rows <= 65536) {
+            if ((r = dalloc(c, &m.tcol16, m.nnz))) return r;
+            KCHK(c, mqi_aux::low16(m.tcol, m.nnz, m.tcol16, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            dfree(m.tcol);
+        }
+    }
+    return 0;
+}
+
 int
 mqi_dij_build(mqi_ctx* c, double rel_thr) {
     int r = ready(c);
@@ -1580,91 +1679,53 @@ mqi_dij_build(mqi_ctx* c, double rel_thr) {
             (r = dalloc(c, &m.val, std::max<int64_t>(1, m.nnz))))
             return r;
         HIPCHK(c, hipMemcpy(m.indptr, indptr.data(), sizeof(int64_t) * (n_spots + 1), hipMemcpyHostToDevice));
-        if ((r = build_plan(c, indptr, m.plan))) return r;
         KCHK(c, mqi_aux::dij_write(c->d_dij_keys, c->d_dij_vals, c->d_dij_off + (size_t) s * n_spots, n_spots, rel_thr,
                                    d_max + (size_t) s * n_spots, m.indptr, m.col, m.val, c->stream));
-        if (m.nnz > 0) {
-            // sort every row by voxel
-            int32_t* tk = nullptr;
-            float*   tv = nullptr;
-            if ((r = dalloc(c, &tk, m.nnz)) || (r = dalloc(c, &tv, m.nnz))) return r;
-            size_t bytes = 0;
-            KCHK(c, mqi_aux::segmented_sort(m.col, m.val, tk, tv, m.nnz, n_spots, m.indptr, m.indptr + 1, nullptr,
-                                            &bytes, c->stream));
-            char* scratch = nullptr;
-            if ((r = dalloc(c, &scratch, std::max<size_t>(bytes, 1)))) return r;
-            KCHK(c, mqi_aux::segmented_sort(m.col, m.val, tk, tv, m.nnz, n_spots, m.indptr, m.indptr + 1, scratch,
-                                            &bytes, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            hipFree(scratch);
-            hipFree(tk);
-            hipFree(tv);
-            // voxel-major copy for D w: row lengths by counting, entries by a
-            // stable sort on the voxel (each voxel's spots ascending)
-            uint32_t* d_cc = nullptr;
-            if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &m.tindptr, m.cols + 1)) ||
-                (r = dalloc(c, &m.tcol, m.nnz)) || (r = dalloc(c, &m.tval, m.nnz)))
-                return r;
-            KCHK(c, mqi_aux::fill_u32(d_cc, 0u, m.cols, c->stream));
-            KCHK(c, mqi_aux::csr_transpose_count(m.col, m.nnz, d_cc, c->stream));
-            std::vector<uint32_t> cc(m.cols);
-            HIPCHK(c, hipMemcpy(cc.data(), d_cc, sizeof(uint32_t) * m.cols, hipMemcpyDeviceToHost));
-            std::vector<int64_t> tptr(m.cols + 1, 0);
-            for (int64_t v = 0; v < m.cols; ++v) tptr[v + 1] = tptr[v] + cc[v];
-            HIPCHK(c, hipMemcpy(m.tindptr, tptr.data(), sizeof(int64_t) * (m.cols + 1), hipMemcpyHostToDevice));
-            if ((r = build_plan(c, tptr, m.tplan))) return r;
-            // compacted gather of D^T g: voxels with a nonzero, and each one's rank
-            std::vector<int32_t> nz, rank(m.cols, 0);
-            for (int64_t v = 0; v < m.cols; ++v)
-                if (cc[v]) {
-                    rank[v] = (int32_t) nz.size();
-                    nz.push_back((int32_t) v);
-                }
-            m.n_nz       = (int64_t) nz.size();
-            int32_t* d_rank = nullptr;
-            if ((r = dalloc(c, &m.nzvox, m.n_nz)) || (r = dalloc(c, &m.ccol, m.nnz)) || (r = dalloc(c, &m.xc, m.n_nz)) ||
-                (r = dalloc(c, &d_rank, m.cols)))
-                return r;
-            HIPCHK(c, hipMemcpy(m.nzvox, nz.data(), sizeof(int32_t) * m.n_nz, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(d_rank, rank.data(), sizeof(int32_t) * m.cols, hipMemcpyHostToDevice));
-            KCHK(c, mqi_aux::renumber(m.col, m.nnz, d_rank, m.ccol, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            hipFree(d_rank);
-            // long rows (the Dij of pencil beams): D^T g over 16-bit columns
-            if (m.nnz >= (int64_t) MQI_SEG_SHORT * m.rows) {
-                if ((r = build_pieces(c, m, indptr))) return r;
-                dfree(m.ccol);
-            }
-            int key_bits = 1;
-            while (key_bits < 32 && (1ll << key_bits) < m.cols) ++key_bits;
-            int32_t*  d_keys = nullptr;
-            uint64_t *d_pk = nullptr, *d_pk2 = nullptr;
-            char*     d_tmp = nullptr;
-            size_t    tbytes = 0;
-            KCHK(c, mqi_aux::csr_transpose_sorted(m.indptr, m.col, m.val, m.rows, m.nnz, key_bits, nullptr, nullptr,
-                                                  nullptr, nullptr, nullptr, nullptr, &tbytes, c->stream));
-            if ((r = dalloc(c, &d_keys, m.nnz)) || (r = dalloc(c, &d_pk, m.nnz)) || (r = dalloc(c, &d_pk2, m.nnz)) ||
-                (r = dalloc(c, &d_tmp, std::max<size_t>(tbytes, 1))))
-                return r;
-            KCHK(c, mqi_aux::csr_transpose_sorted(m.indptr, m.col, m.val, m.rows, m.nnz, key_bits, d_keys, d_pk, d_pk2,
-                                                  m.tcol, m.tval, d_tmp, &tbytes, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            hipFree(d_keys);
-            hipFree(d_pk);
-            hipFree(d_pk2);
-            hipFree(d_tmp);
-            hipFree(d_cc);
-            // D w over 16-bit spot columns
-            if (m.rows <= 65536) {
-                if ((r = dalloc(c, &m.tcol16, m.nnz))) return r;
-                KCHK(c, mqi_aux::low16(m.tcol, m.nnz, m.tcol16, c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                dfree(m.tcol);
-            }
-        }
+        if ((r = prepare_products(c, m, indptr, true))) return r;
     }
     hipFree(d_max);
     hipFree(d_cnt);
+    return 0;
+}
+
+int
+mqi_dij_set_csr(mqi_ctx* c, int s, int64_t rows, int64_t cols, const int64_t* indptr, const int32_t* indices,
+                const float* values) {
+    if (!c) return MQI_E_ARG;
+    int r = 0;
+    if (s < 0 || s >= (1 << 16)) return fail(c, MQI_E_ARG, "scenario %d out of range", s);
+    if (rows < 0 || rows >= (1ll << 31) || cols < 1 || cols >= (1ll << 31))
+        return fail(c, MQI_E_ARG, "rows %lld / cols %lld out of range", (long long) rows, (long long) cols);
+    if (!indptr) return fail(c, MQI_E_ARG, "indptr is NULL");
+    const int64_t nnz = indptr[rows];
+    if (nnz > 0 && (!indices || !values)) return fail(c, MQI_E_ARG, "indices or values are NULL");
+    int64_t bad = 0;
+    const int e = check_csr(rows, cols, indptr, indices, &bad);
+    if (e == 1) return fail(c, MQI_E_ARG, "indptr must start at 0 and not decrease (row %lld)", (long long) bad);
+    if (e == 2) return fail(c, MQI_E_ARG, "column out of [0, cols) at entry %lld", (long long) bad);
+    if (e == 3) return fail(c, MQI_E_ARG, "columns must ascend strictly within a row (entry %lld)", (long long) bad);
+    if ((r = dij_size_check(nnz)))
+        return fail(c, r, "%lld nonzeros: at most %lld", (long long) nnz, (long long) MQI_MAX_SORT_NNZ);
+    hipSetDevice(c->device);
+    if ((int) c->dij.size() <= s) c->dij.resize(s + 1);
+    csr_dev& m = c->dij[s];
+    free_csr(m);
+    m.rows = rows;
+    m.cols = cols;
+    m.nnz  = nnz;
+    if ((r = dalloc(c, &m.indptr, rows + 1)) || (r = dalloc(c, &m.col, std::max<int64_t>(1, nnz))) ||
+        (r = dalloc(c, &m.val, std::max<int64_t>(1, nnz))))
+        return r;
+    std::vector<int64_t> ip(indptr, indptr + rows + 1);
+    HIPCHK(c, hipMemcpy(m.indptr, ip.data(), sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCHK(c, hipMemcpy(m.col, indices, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(m.val, values, sizeof(float) * nnz, hipMemcpyHostToDevice));
+    }
+    if ((r = prepare_products(c, m, ip, false))) {
+        free_csr(m);
+        return r;
+    }
     return 0;
 }
 

@@ -9,7 +9,8 @@
 #include <algorithm>
 #include <vector>
 
-// Pieces of the D^T g product over 16-bit column deltas: each row of a CSR
+// Pieces of the D^T g product over column deltas (each at most 65535 within
+// a piece): each row of a CSR
 // (row pointers ip[0..rows]) is cut at its first nonzero, at every position
 // in `gaps` (ascending; nonzeros whose column is more than 65535 past the
 // previous one of the row; positions at a row start are ignored) and every
@@ -41,6 +42,38 @@ cut_pieces(const std::vector<int64_t>& ip, const std::vector<int64_t>& gaps, int
     row_piece[rows] = (int32_t) pk.size();
     pk.push_back(rows >= 0 ? ip[rows] : 0);
     return true;
+}
+
+// A CSR handed in by a caller (rows x cols, row pointers ip[0..rows], column
+// indices idx[0 .. ip[rows])): 0 when ip starts at 0 and does not decrease
+// and every row's columns lie in [0, cols) and ascend strictly; else 1 (ip,
+// *bad = the row), 2 (a column out of range) or 3 (columns not ascending),
+// *bad = the entry.
+static inline int
+check_csr(int64_t rows, int64_t cols, const int64_t* ip, const int32_t* idx, int64_t* bad) {
+    // the row pointers first: idx is read only up to ip[rows]
+    if (ip[0] != 0) {
+        *bad = 0;
+        return 1;
+    }
+    for (int64_t i = 0; i < rows; ++i)
+        if (ip[i + 1] < ip[i]) {
+            *bad = i;
+            return 1;
+        }
+    for (int64_t i = 0; i < rows; ++i) {
+        for (int64_t k = ip[i]; k < ip[i + 1]; ++k) {
+            if (idx[k] < 0 || (int64_t) idx[k] >= cols) {
+                *bad = k;
+                return 2;
+            }
+            if (k > ip[i] && idx[k] <= idx[k - 1]) {
+                *bad = k;
+                return 3;
+            }
+        }
+    }
+    return 0;
 }
 
 #endif

@@ -244,3 +244,105 @@ def test_forward_product_is_reproducible_across_builds(ctx):
         assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
     D = _csr(ctx)
     assert np.allclose(outs[0], D.T @ w.cpu().numpy().astype(np.float64), rtol=1e-9, atol=0)
+
+
+def _adversarial_long(rng):
+    """Long rows (the pieces form of D^T g): empty and single-entry rows,
+    runs of consecutive columns, deltas at and around the one-byte escape
+    (254 / 255 / 256), rows whose every delta is an escape (several escape
+    reloads per batch), deltas of exactly 65535 and steps above it (new
+    pieces), lengths around the 256-nonzero batch and 4096-nonzero piece."""
+    cols = 2_000_000
+    rows = []
+    rows.append(np.zeros(0, np.int64))
+    rows.append(np.array([cols - 1]))
+    rows.append(np.arange(4096) + 17)
+    rows.append(np.arange(5000) * 300 + 3)
+    rows.append(np.cumsum(np.tile([254, 255, 256, 1], 750)))
+    rows.append(np.concatenate([np.arange(500) + k * 100_000 for k in range(10)]))
+    rows.append(np.sort(rng.choice(cols, 257, replace=False)))
+    rows.append(np.sort(rng.choice(cols, 12289, replace=False)))
+    rows.append(np.arange(30) * 65535 + 5)
+    rows.append(np.arange(20000) + 700_000)
+    d = rng.geometric(0.3, 9000)
+    far = rng.random(d.size) < 0.03
+    d[far] = rng.integers(255, 70_000, int(far.sum()))
+    rows.append(np.cumsum(d))
+    rows.append(np.cumsum(np.r_[np.ones(2048, np.int64) * 300, np.ones(2049, np.int64)]))
+    rows = [r[r < cols] for r in rows]
+    return rows, cols
+
+
+def _to_csr(rows, cols, rng):
+    indptr = np.r_[0, np.cumsum([len(r) for r in rows])].astype(np.int64)
+    indices = np.concatenate(rows).astype(np.int32) if indptr[-1] else np.zeros(0, np.int32)
+    values = rng.uniform(1e-6, 1.0, indices.size).astype(np.float32)
+    return sps.csr_matrix((values, indices, indptr), shape=(len(rows), cols))
+
+
+def _check_products(ctx, D, scen, rng):
+    import torch
+    dev = torch.device("cuda", 0)
+    w = rng.uniform(0.5, 1.5, D.shape[0]).astype(np.float32)
+    g = rng.normal(size=D.shape[1]).astype(np.float32)
+    tw, tg = torch.from_numpy(w).to(dev), torch.from_numpy(g).to(dev)
+    y0 = rng.normal(size=D.shape[0])
+    grad = torch.from_numpy(y0.copy()).to(dev)
+    ctx.dij_matvec(scen, 1, tg.data_ptr(), grad.data_ptr(), y_double=True, alpha=1.5, accumulate=True)
+    dose = torch.zeros(D.shape[1], dtype=torch.float32, device=dev)
+    ctx.dij_matvec(scen, 0, tw.data_ptr(), dose.data_ptr(), y_double=False)
+    torch.cuda.synchronize()
+    D64 = D.astype(np.float64)
+    want_grad = y0 + 1.5 * (D64 @ g.astype(np.float64))
+    want_dose = D64.T @ w.astype(np.float64)
+    got_grad, got_dose = grad.cpu().numpy(), dose.cpu().numpy()
+    assert np.allclose(got_grad, want_grad, rtol=1e-6, atol=1e-9 * max(1.0, np.abs(want_grad).max()))
+    assert np.allclose(got_dose, want_dose, rtol=2e-6, atol=1e-6 * max(1e-30, np.abs(want_dose).max()))
+    return got_grad, got_dose
+
+
+@pytest.mark.parametrize("kind", ["long", "short", "many_rows"])
+def test_set_csr_products_match_fp64(ctx, kind, tmp_path):
+    """A caller's CSR through mqi_dij_set_csr: both products against scipy
+    in FP64 on matrices built to reach every path of the kernels (long rows:
+    the byte-delta pieces with escapes, escape reloads and 65535-column
+    steps; short rows: the compacted CSR-Adaptive form; more than 65536 rows:
+    32-bit spot columns in D w), the matrix read back unchanged, and the
+    same matrix through the .npz writer and loader giving the same results
+    bit for bit."""
+    rng = np.random.default_rng({"long": 1, "short": 2, "many_rows": 3}[kind])
+    if kind == "long":
+        rows, cols = _adversarial_long(rng)
+        D = _to_csr(rows, cols, rng)
+        assert D.nnz >= 1024 * D.shape[0]
+    elif kind == "short":
+        D = sps.random(3000, 100_000, density=2e-4, format="csr", dtype=np.float32, random_state=4)
+        D.data = rng.uniform(1e-6, 1.0, D.nnz).astype(np.float32)
+    else:
+        D = sps.random(70_000, 5000, density=4e-4, format="csr", dtype=np.float32, random_state=5)
+        D.data = rng.uniform(1e-6, 1.0, D.nnz).astype(np.float32)
+    D.sort_indices()
+    ctx.dij_set_csr(D.indptr, D.indices, D.data, D.shape, scenario=0)
+    assert ctx.dij_info(0) == (D.nnz, D.shape[0], D.shape[1])
+    ip, ix, v, shape = ctx.dij_csr(0)
+    assert shape == D.shape and np.array_equal(ip, D.indptr) and np.array_equal(ix, D.indices)
+    assert np.array_equal(v.view(np.uint32), D.data.view(np.uint32))
+    a = _check_products(ctx, D, 0, np.random.default_rng(9))
+    path = str(tmp_path / "d.npz")
+    ctx.dij_save_npz(path, 0)
+    ctx.dij_load_npz(path, scenario=1)
+    b = _check_products(ctx, D, 1, np.random.default_rng(9))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_set_csr_rejects_malformed(ctx):
+    """Unsorted or repeated columns, columns out of range and row pointers
+    that do not start at 0 or decrease are argument errors naming the entry."""
+    from moquimc_amd._abi import MqiError
+    ip = np.array([0, 2, 4], np.int64)
+    v = np.ones(4, np.float32)
+    for idx, ipp, msg in [([1, 0, 2, 3], ip, "ascend"), ([0, 1, 2, 2], ip, "ascend"), ([0, 1, 2, 10], ip, "cols"),
+                          ([0, 1, 2, -1], ip, "cols"), ([0, 1, 2, 3], np.array([1, 2, 4], np.int64), "indptr"),
+                          ([0, 1, 2, 3], np.array([0, 3, 2], np.int64), "indptr")]:
+        with pytest.raises(MqiError, match=msg):
+            ctx.dij_set_csr(ipp, np.array(idx, np.int32), v, (2, 10))

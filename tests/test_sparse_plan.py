@@ -85,3 +85,51 @@ def test_cut_pieces_tile_rows(tmp_path):
             if s in starts or s in inner_gaps or k in empty:
                 continue
             assert s - pk[k - 1] == plen
+
+
+CHECK_DRIVER = r"""
+#include <cstdio>
+#include <vector>
+#include "mqi_plan.h"
+int main() {
+    long long rows, cols, nnz;
+    while (std::scanf("%lld %lld %lld", &rows, &cols, &nnz) == 3) {
+        std::vector<int64_t> ip(rows + 1);
+        std::vector<int32_t> idx(nnz);
+        for (auto& v : ip)
+            if (std::scanf("%lld", (long long*) &v) != 1) return 1;
+        for (auto& v : idx)
+            if (std::scanf("%d", &v) != 1) return 1;
+        int64_t bad = -1;
+        const int e = check_csr(rows, cols, ip.data(), idx.data(), &bad);
+        std::printf("%d %lld\n", e, (long long) bad);
+    }
+    return 0;
+}
+"""
+
+
+def test_check_csr_codes(tmp_path):
+    """The validation of a caller's CSR (check_csr): each kind of defect is
+    reported with its code and the first row or entry that has it."""
+    src = tmp_path / "check.cpp"
+    src.write_text(CHECK_DRIVER)
+    exe = str(tmp_path / "check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "moquimc_amd", "csrc"),
+                    str(src), "-o", exe], check=True)
+    cases = [  # rows, cols, indptr, indices, (code, bad)
+        (2, 10, [0, 2, 4], [1, 5, 0, 9], (0, -1)),
+        (3, 10, [0, 0, 0, 0], [], (0, -1)),
+        (2, 10, [1, 2, 4], [1, 5, 0, 9], (1, 0)),
+        (2, 10, [0, 3, 2], [1, 5, 7], (1, 1)),
+        (2, 10, [0, 2, 4], [1, 5, 0, 10], (2, 3)),
+        (2, 10, [0, 2, 4], [-1, 5, 0, 9], (2, 0)),
+        (2, 10, [0, 2, 4], [5, 5, 0, 9], (3, 1)),
+        (2, 10, [0, 2, 4], [1, 5, 9, 0], (3, 3)),
+        (1, 3, [0, 3], [0, 1, 2], (0, -1)),
+    ]
+    inp = "".join("%d %d %d %s %s\n" % (r, c, len(ix), " ".join(map(str, ip)), " ".join(map(str, ix)))
+                  for r, c, ip, ix, _ in cases)
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout.split()
+    got = [(int(out[2 * k]), int(out[2 * k + 1])) for k in range(len(cases))]
+    assert got == [want for *_, want in cases]
