@@ -3,7 +3,8 @@ kernels):
 
 - the 5000-spot Dij plan (CT 256x256x128, 1e8 histories): the CSR rows sum
   to the dense dose of the same run, and D^T g and D w equal scipy's FP64
-  products of the copied matrix;
+  products of the copied matrix; the quadratic objective's gradient through
+  the autograd layer equals scipy's FP64 gradient and central differences;
 - the 2000-spot CT plan (512x512x200, 4.27e7 histories): dose is additive
   over spots (every history's random stream is keyed by seed, spot and
   history, so the plan's dose equals the sum of the doses of its even and odd
@@ -47,6 +48,52 @@ def test_full_dij_rows_sum_to_dose_and_products_match_fp64(ctx):
     want_d = D.T @ wt.astype(np.float64)
     assert np.allclose(y.cpu().numpy(), want_y, rtol=1e-9, atol=1e-12 * np.abs(want_y).max())
     assert np.allclose(d.cpu().numpy(), want_d, rtol=1e-9, atol=1e-12 * np.abs(want_d).max())
+
+
+def test_full_dij_quadratic_gradient_and_finite_differences(ctx):
+    """The 5000-spot Dij (threshold 1e-4 of each spot's maximum): the
+    quadratic objective against a prescription in a 60 mm sphere through the
+    autograd layer; its value and gradient against scipy in FP64 (gradient
+    within 1e-5 of the largest component), and central differences on 64
+    random spot weights (exact for a quadratic, up to rounding)."""
+    import torch
+    from moquimc_amd import dij, workloads
+    w = workloads.dij()
+    w.load(ctx)
+    res = ctx.run(ctx.config(seed=23, batched=True, dij=True, dij_capacity=1 << 19))
+    assert res.status == 0
+    ctx.dij_build(1e-4)
+    indptr, idx, val, shape = ctx.dij_csr()
+    D = sps.csr_matrix((val.astype(np.float64), idx, indptr), shape=shape).T.tocsr()   # voxels x spots
+    mat = dij.DijMatrix(ctx, 0)
+    rng = np.random.default_rng(17)
+    w_np = rng.uniform(0.5, 1.5, mat.n_spots).astype(np.float32)
+    # 2 Gy in the target, expressed in the Dij's units: the unit-weight plan's
+    # mean dose in the target maps to 2
+    inside = workloads.target_prescription(w.xe, w.ye, w.ze, radius=30.0, dose=1.0) > 0
+    level = float((D @ np.ones(mat.n_spots))[inside].mean())
+    p_np = np.where(inside, level, 0.0)
+    dev = torch.device("cuda", 0)
+    wt = torch.tensor(w_np, device=dev, requires_grad=True)
+    p = torch.tensor(p_np, device=dev)
+    f = dij.quadratic_objective(mat, wt, p)
+    f.backward()
+    r = D @ w_np.astype(np.float64) - p_np
+    f_ref = float(np.mean(r * r))
+    g_ref = 2.0 * (D.T @ r) / D.shape[0]
+    got = wt.grad.detach().cpu().numpy().astype(np.float64)
+    assert abs(f.item() - f_ref) <= 1e-9 * abs(f_ref)
+    assert np.abs(got - g_ref).max() <= 1e-5 * np.abs(g_ref).max()
+    h = np.float32(2.0 ** -4)
+    for j in rng.choice(mat.n_spots, 64, replace=False):
+        wp, wm = w_np.copy(), w_np.copy()
+        wp[j] += h
+        wm[j] -= h
+        with torch.no_grad():
+            fp = dij.quadratic_objective(mat, torch.tensor(wp, device=dev), p).item()
+            fm = dij.quadratic_objective(mat, torch.tensor(wm, device=dev), p).item()
+        fd = (fp - fm) / (float(wp[j]) - float(wm[j]))
+        assert abs(fd - got[j]) <= 1e-6 * np.abs(g_ref).max() + 1e-5 * abs(got[j]), (j, fd, got[j])
 
 
 def test_full_ct_plan_dose_is_additive_over_spots(ctx):
