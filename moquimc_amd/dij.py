@@ -83,7 +83,14 @@ def quadratic_objective(mat, w, prescription, voxel_weights=None):
 
 
 def worst_case_objective(mats, w, prescription, voxel_weights=None):
-    """max over scenarios of the quadratic objective, and the per-scenario
-    values (the gradient flows through the worst scenario)."""
-    vals = torch.stack([quadratic_objective(m, w, prescription, voxel_weights) for m in mats])
-    return vals.max(), vals
+    """max over scenarios of the quadratic objective (differentiable: the
+    gradient is the worst scenario's), and the per-scenario values (a
+    detached vector).  The values are evaluated without a graph and only the
+    worst scenario's objective is formed again with one, so a backward pass
+    runs one D^T g and not one per scenario (a graph over every scenario
+    would carry zero gradients through the others).  The products are
+    deterministic: the returned maximum equals its entry in the values."""
+    with torch.no_grad():
+        vals = torch.stack([quadratic_objective(m, w, prescription, voxel_weights) for m in mats])
+    k = int(torch.argmax(vals).item())
+    return quadratic_objective(mats[k], w, prescription, voxel_weights), vals
