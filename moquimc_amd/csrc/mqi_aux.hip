@@ -342,6 +342,9 @@ template<typename TC> struct col4;
 template<> struct col4<int32_t> { typedef i32x4 type; };
 template<> struct col4<uint16_t> { typedef u16x4 type; };
 
+// a packed segment is loaded as at most 4 x 64 vectors of four entries
+static_assert(MQI_SEG_SHORT <= 1024, "packed segments are loaded as at most 256 vectors");
+
 template<typename TY, typename TC>
 __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__ indptr,
                                                      const TC* __restrict__ col,
@@ -397,20 +400,40 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             if (lane == 0) part[s] = acc;
         } else {
             const int64_t r0 = info, r1 = seg_info[s + 1] & 0x7fffffffu;
-            // products into LDS, four coalesced loads in flight per lane
-            int64_t k = kb + lane;
-            for (; k + 192 < ke; k += 256) {
-                float   v[4];
-                int32_t cc[4];   // TC widened
+            // products into LDS: the segment's aligned body as 16-byte loads
+            // (four consecutive entries per lane, the whole segment in flight
+            // at once: at most 1024 entries, 256 vectors), head and tail of
+            // up to three entries one per lane
+            typedef typename col4<TC>::type c4t;
+            int64_t ka = (kb + 3) & ~(int64_t) 3;
+            if (ka > ke) ka = ke;
+            const int64_t n4 = (ke - ka) >> 2;
+            const int64_t kt = ka + (n4 << 2);
+            const f32x4*  v4 = reinterpret_cast<const f32x4*>(val + ka);
+            const c4t*    c4 = reinterpret_cast<const c4t*>(col + ka);
+            f32x4         v[4];
+            c4t           c[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    v[u]  = val[k + 64 * u];
-                    cc[u] = col[k + 64 * u];
+            for (int u = 0; u < 4; ++u) {
+                const int64_t i = lane + 64 * u;
+                if (i < n4) {
+                    v[u] = __builtin_nontemporal_load(v4 + i);
+                    c[u] = __builtin_nontemporal_load(c4 + i);
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) buf[k + 64 * u - kb] = (double) v[u] * (double) x[cc[u]];
             }
-            for (; k < ke; k += 64) buf[k - kb] = (double) val[k] * (double) x[col[k]];
+            if (lane < ka - kb) buf[lane] = (double) val[kb + lane] * (double) x[col[kb + lane]];
+            if (lane < ke - kt) buf[kt - kb + lane] = (double) val[kt + lane] * (double) x[col[kt + lane]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t i = lane + 64 * u;
+                if (i < n4) {
+                    double* b = buf + (ka - kb) + 4 * i;
+                    b[0]      = (double) v[u].x * (double) x[c[u].x];
+                    b[1]      = (double) v[u].y * (double) x[c[u].y];
+                    b[2]      = (double) v[u].z * (double) x[c[u].z];
+                    b[3]      = (double) v[u].w * (double) x[c[u].w];
+                }
+            }
             __syncthreads();
             // L lanes per row (a power of two, L * rows <= 64): each sums a
             // strided part, then the group adds its parts with shuffles
