@@ -421,8 +421,8 @@ build_pieces(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& ip) {
     KCHK(c, mqi_aux::stage_pieces(m.val, d_dcol, d_pk, d_pb, d_ecnt, m.n_piece, m.sval, m.sd8, m.pesc, c->stream));
     // processed in the order of their first columns: the pieces that run at
     // the same time gather from the same stretch of x (7 % faster on the
-    // 5000-spot Dij than row order); one 16-byte header per piece in that
-    // order, so a wave starts a piece after one load
+    // 5000-spot Dij than row order); one 32-byte header per piece in that
+    // order, so a wave starts a piece after one round trip
     std::vector<int32_t> first(m.n_piece), order(m.n_piece);
     if (m.n_piece) HIPCHK(c, hipMemcpy(first.data(), d_base, sizeof(int32_t) * m.n_piece, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < m.n_piece; ++i) order[i] = (int32_t) i;
@@ -1610,7 +1610,7 @@ prepare_products(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& indptr, boo
         KCHK(c, mqi_aux::renumber(m.col, m.nnz, d_rank, m.ccol, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         hipFree(d_rank);
-        // long rows (the Dij of pencil beams): D^T g over 16-bit columns
+        // long rows (the Dij of pencil beams): D^T g over byte column deltas
         if (m.nnz >= (int64_t) MQI_SEG_SHORT * m.rows) {
             if ((r = build_pieces(c, m, indptr))) return r;
             dfree(m.ccol);
