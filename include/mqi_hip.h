@@ -207,6 +207,17 @@ int mqi_dij_device_ptrs(mqi_ctx* ctx, int scenario, const int64_t** indptr, cons
 int mqi_dij_matvec(mqi_ctx* ctx, int scenario, int transpose, const float* x, void* y, int y_double, double alpha,
                    int accumulate, void* stream);
 
+/* The quadratic objective of the dose d = alpha * D w of scenario `scenario`
+ * in one pass over the D w matrix, without d: *f = mean_v c_v (d_v - p_v)^2
+ * (device double; c NULL: all 1) and, when g is set, g_v = 2 c_v (d_v - p_v) /
+ * n_voxels (device float32), the input of the D^T g that gives the gradient
+ * alpha D^T g.  w: device float32 (spots); p: device double (voxels).
+ * Enqueued on stream like mqi_dij_matvec, in a fixed order.  No counterpart
+ * in the reference (its Dij is only written out): the objective of the
+ * differentiable layer over the Dij. */
+int mqi_dij_quadratic(mqi_ctx* ctx, int scenario, const float* w, const double* p, const float* c, double alpha,
+                      float* g, double* f, void* stream);
+
 /* Region of interest of the scorers (target 0: Dose, LETd, LETt, Edep, Dm
  * and Dij) or of the statistics pair (target 1), from a voxel mask (x
  * fastest, the grid's size; the reference's mask_total, the sum of the

@@ -84,6 +84,7 @@ def lib():
                                  ctypes.POINTER(ctypes.c_int64)]),
             "mqi_dij_copy": (i, [vp, i, vp, vp, vp]),
             "mqi_dij_set_csr": (i, [vp, i, ctypes.c_int64, ctypes.c_int64, vp, vp, vp]),
+            "mqi_dij_quadratic": (i, [vp, i, vp, vp, vp, d, vp, vp, vp]),
             "mqi_dij_device_ptrs": (i, [vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
             "mqi_dij_matvec": (i, [vp, i, i, vp, vp, i, d, i, vp]),
             "mqi_last_matvec_ms": (i, [vp, ctypes.POINTER(d)]),
@@ -108,7 +109,7 @@ def lib():
 EXPORTED = ["mqi_create", "mqi_destroy", "mqi_last_error", "mqi_device_info", "mqi_set_grid", "mqi_set_grid_hu",
             "mqi_get_density", "mqi_set_spots", "mqi_set_scenarios", "mqi_set_scorers", "mqi_zero_scorers",
             "mqi_scorer_count", "mqi_run", "mqi_run_vertices", "mqi_trace", "mqi_get_scorer",
-            "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy", "mqi_dij_set_csr",
+            "mqi_scorer_device_ptr", "mqi_stat", "mqi_dij_build", "mqi_dij_info", "mqi_dij_copy", "mqi_dij_set_csr", "mqi_dij_quadratic",
             "mqi_dij_device_ptrs", "mqi_dij_matvec", "mqi_last_matvec_ms", "mqi_math_probe", "mqi_autotune",
             "mqi_dij_save_npz", "mqi_save_scorer", "mqi_set_grid_ct_mha", "mqi_set_beamline", "mqi_set_roi", "mqi_set_roi_mha",
             "mqi_physics_probe", "mqi_run_vertices_noalloc", "mqi_atomic_probe"]
@@ -401,6 +402,15 @@ class Context:
                                          ctypes.c_void_p(y_ptr), int(y_double), ctypes.c_double(alpha),
                                          int(accumulate), None if stream is None else ctypes.c_void_p(stream)),
                   "mqi_dij_matvec")
+
+    def dij_quadratic(self, scenario, w_ptr, p_ptr, c_ptr, alpha, g_ptr, f_ptr, stream=None):
+        """f = mean c (alpha D w - p)^2 and g = 2 c (alpha D w - p) / n on the
+        device (mqi_dij_quadratic; c_ptr / g_ptr may be 0)."""
+        self._chk(self._L.mqi_dij_quadratic(self.h, scenario, ctypes.c_void_p(w_ptr), ctypes.c_void_p(p_ptr),
+                                            ctypes.c_void_p(c_ptr or None), ctypes.c_double(alpha),
+                                            ctypes.c_void_p(g_ptr or None), ctypes.c_void_p(f_ptr),
+                                            None if stream is None else ctypes.c_void_p(stream)),
+                  "mqi_dij_quadratic")
 
     def last_matvec_ms(self):
         ms = ctypes.c_double()

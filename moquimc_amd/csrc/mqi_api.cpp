@@ -34,6 +34,7 @@ struct csr_plan {
     uint32_t* seg_info  = nullptr;   // n_seg + 1
     int32_t*  split_seg = nullptr;   // 2 per split row
     double*   part      = nullptr;   // n_seg
+    double*   qpart     = nullptr;   // n_seg + n_split: terms of the quadratic objective (allocated on first use)
 };
 
 struct csr_dev {
@@ -275,6 +276,7 @@ free_csr(csr_dev& m) {
             dfree(p->seg_info);
             dfree(p->split_seg);
             dfree(p->part);
+            dfree(p->qpart);
         }
     }
     m = csr_dev();
@@ -1956,6 +1958,38 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
             KCHK(c, mqi_aux::csr_adaptive(m.tindptr, m.tcol, m.tval, x, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
                                           p.n_split, p.part, y, y_double, alpha, accumulate, st));
     }
+    HIPCHK(c, hipEventRecord(c->ev_mv1, st));
+    c->matvec_pending = true;
+    return 0;
+}
+
+int
+mqi_dij_quadratic(mqi_ctx* c, int s, const float* w, const double* p, const float* cw, double alpha, float* g,
+                  double* f, void* stream) {
+    if (!c || !w || !p || !f) return MQI_E_ARG;
+    if (s < 0 || s >= (int) c->dij.size()) return fail(c, MQI_E_STATE, "no Dij for scenario %d", s);
+    hipSetDevice(c->device);
+    csr_dev& m = c->dij[s];
+    if (m.nnz == 0) return fail(c, MQI_E_STATE, "scenario %d's Dij has no nonzeros", s);
+    csr_plan& P = m.tplan;
+    int       r = 0;
+    if (!P.qpart && (r = dalloc(c, &P.qpart, std::max<int64_t>(P.n_seg + P.n_split, 1)))) return r;
+    hipStream_t       st = stream ? (hipStream_t) stream : c->stream;
+    mqi_aux::quad_obj o;
+    o.p         = p;
+    o.c         = cw;
+    o.g         = g;
+    o.part      = P.qpart;
+    o.two_inv_n = 2.0 / (double) m.cols;
+    o.n_seg     = P.n_seg;
+    const double inv_n = 1.0 / (double) m.cols;
+    HIPCHK(c, hipEventRecord(c->ev_mv0, st));
+    if (m.tcol16)
+        KCHK(c, mqi_aux::csr_quadratic16(m.tindptr, m.tcol16, m.tval, w, P.seg_k, P.seg_info, P.n_seg, P.split_seg,
+                                         P.n_split, P.part, alpha, o, inv_n, f, st));
+    else
+        KCHK(c, mqi_aux::csr_quadratic(m.tindptr, m.tcol, m.tval, w, P.seg_k, P.seg_info, P.n_seg, P.split_seg,
+                                       P.n_split, P.part, alpha, o, inv_n, f, st));
     HIPCHK(c, hipEventRecord(c->ev_mv1, st));
     c->matvec_pending = true;
     return 0;

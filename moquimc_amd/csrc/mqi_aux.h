@@ -42,6 +42,25 @@ int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, co
 int csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
                    const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
                    void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
+// The quadratic objective of y = alpha * A x on the D w matrix, without y:
+// f = inv_n * sum_r c_r (y_r - p_r)^2 (c NULL: 1) into *f (device), and
+// g_r = two_inv_n * c_r (y_r - p_r) (float32, when g is set).  part: n_seg +
+// n_split doubles of per-segment and per-split-row terms, summed in a fixed
+// order.
+struct quad_obj {
+    const double* p;
+    const float*  c;
+    float*        g;
+    double*       part;
+    double        two_inv_n;
+    int64_t       n_seg;
+};
+int csr_quadratic(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
+                  const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+                  double alpha, quad_obj o, double inv_n, double* f, hipStream_t s);
+int csr_quadratic16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x,
+                    const int64_t* seg_k, const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg,
+                    int64_t n_split, double* part, double alpha, quad_obj o, double inv_n, double* f, hipStream_t s);
 // D^T g over column deltas (mqi_aux.hip): the 16-bit deltas of each row's
 // columns (0 at row starts and where a row skips more than 65535 columns,
 // those positions listed in gaps), the column before each piece's first

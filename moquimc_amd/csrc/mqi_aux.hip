@@ -345,14 +345,18 @@ template<> struct col4<uint16_t> { typedef u16x4 type; };
 // a packed segment is loaded as at most 4 x 64 vectors of four entries
 static_assert(MQI_SEG_SHORT <= 1024, "packed segments are loaded as at most 256 vectors");
 
-template<typename TY, typename TC>
+// OBJ: instead of y, the quadratic objective of y (quad_obj): per row
+// c_r (y_r - p_r)^2 summed over each packed segment into o.part[s] (0 for a
+// long-row segment, whose rows k_csr_split_rows finishes), and
+// o.g[r] = two_inv_n c_r (y_r - p_r) when o.g is set
+template<typename TY, typename TC, bool OBJ>
 __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__ indptr,
                                                      const TC* __restrict__ col,
                                                      const float* __restrict__ val, const float* __restrict__ x,
                                                      const int64_t* __restrict__ seg_k,
                                                      const uint32_t* __restrict__ seg_info, int64_t n_seg,
                                                      double* __restrict__ part, TY* __restrict__ y, double alpha,
-                                                     int accumulate) {
+                                                     int accumulate, quad_obj o) {
     __shared__ double buf[MQI_SEG_SHORT];
     const int lane = threadIdx.x;
     // plain grid-stride walk: a remap giving each XCD a contiguous run of
@@ -397,7 +401,10 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             const int64_t kt = ka + (n4 << 2);
             if (lane < ke - kt) acc += (double) val[kt + lane] * (double) x[col[kt + lane]];
             for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-            if (lane == 0) part[s] = acc;
+            if (lane == 0) {
+                part[s] = acc;
+                if (OBJ) o.part[s] = 0.0;
+            }
         } else {
             const int64_t r0 = info, r1 = seg_info[s + 1] & 0x7fffffffu;
             // products into LDS: the segment's aligned body as 16-byte loads
@@ -441,6 +448,7 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             int           L  = 1;
             while (L < 64 && 2 * L * nr <= 64) L *= 2;
             const int sub = lane & (L - 1);
+            double    obj = 0.0;   // this lane's rows' objective terms, in row order
             for (int64_t rb = r0; rb < r1; rb += 64 / L) {
                 const int64_t r   = rb + lane / L;
                 double        acc = 0.0;
@@ -451,8 +459,19 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
                 for (int off = L >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
                 if (r < r1 && sub == 0) {
                     const double out = alpha * acc;
-                    y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+                    if (OBJ) {
+                        const double res = out - o.p[r];
+                        const double cw  = o.c ? (double) o.c[r] : 1.0;
+                        obj += cw * res * res;
+                        if (o.g) o.g[r] = (float) (o.two_inv_n * cw * res);
+                    } else {
+                        y[r] = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+                    }
                 }
+            }
+            if (OBJ) {
+                for (int off = 32; off > 0; off >>= 1) obj += __shfl_xor(obj, off);
+                if (lane == 0) o.part[s] = obj;
             }
             __syncthreads();
         }
@@ -461,10 +480,10 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
 
 // rows split over partial segments [split_seg[2i], split_seg[2i+1]):
 // y[r] = alpha * sum of the pieces in order (+ y[r])
-template<typename TY>
+template<typename TY, bool OBJ>
 __global__ void k_csr_split_rows(const uint32_t* __restrict__ seg_info, const int32_t* __restrict__ split_seg,
                                  int64_t n_split, const double* __restrict__ part, TY* __restrict__ y, double alpha,
-                                 int accumulate) {
+                                 int accumulate, quad_obj o) {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_split) return;
     const int32_t s0 = split_seg[2 * i], s1 = split_seg[2 * i + 1];
@@ -472,7 +491,25 @@ __global__ void k_csr_split_rows(const uint32_t* __restrict__ seg_info, const in
     double        acc = 0.0;
     for (int32_t s = s0; s < s1; ++s) acc += part[s];
     const double out = alpha * acc;
-    y[r]             = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+    if (OBJ) {   // the row's objective term after the packed segments' ones
+        const double res = out - o.p[r];
+        const double cw  = o.c ? (double) o.c[r] : 1.0;
+        o.part[o.n_seg + i] = cw * res * res;
+        if (o.g) o.g[r] = (float) (o.two_inv_n * cw * res);
+    } else {
+        y[r] = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+    }
+}
+
+// f = inv_n * (o.part[0] + ... + o.part[n - 1]) in a fixed order (one
+// workgroup: strided partial sums, then a block reduction)
+__global__ void k_quad_sum(const double* __restrict__ part, int64_t n, double inv_n, double* __restrict__ f) {
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += part[i];
+    typedef hipcub::BlockReduce<double, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    const double t = BR(tmp).Sum(acc);
+    if (threadIdx.x == 0) *f = inv_n * t;
 }
 
 template<typename TC>
@@ -482,21 +519,39 @@ csr_adaptive_t(const int64_t* indptr, const TC* col, const float* val, const flo
                void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
     if (n_seg <= 0) return 0;
     const int blocks = (int) (n_seg < 262144 ? n_seg : 262144);
+    const quad_obj o{};
     if (y_is_double)
-        hipLaunchKernelGGL((k_csr_adaptive<double, TC>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k,
-                           seg_info, n_seg, part, (double*) y, alpha, accumulate);
+        hipLaunchKernelGGL((k_csr_adaptive<double, TC, false>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x,
+                           seg_k, seg_info, n_seg, part, (double*) y, alpha, accumulate, o);
     else
-        hipLaunchKernelGGL((k_csr_adaptive<float, TC>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k,
-                           seg_info, n_seg, part, (float*) y, alpha, accumulate);
+        hipLaunchKernelGGL((k_csr_adaptive<float, TC, false>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x,
+                           seg_k, seg_info, n_seg, part, (float*) y, alpha, accumulate, o);
     if (n_split > 0) {
         const int b = (int) ((n_split + 255) / 256);
         if (y_is_double)
-            hipLaunchKernelGGL(k_csr_split_rows<double>, dim3(b), dim3(256), 0, s, seg_info, split_seg, n_split, part,
-                               (double*) y, alpha, accumulate);
+            hipLaunchKernelGGL((k_csr_split_rows<double, false>), dim3(b), dim3(256), 0, s, seg_info, split_seg,
+                               n_split, part, (double*) y, alpha, accumulate, o);
         else
-            hipLaunchKernelGGL(k_csr_split_rows<float>, dim3(b), dim3(256), 0, s, seg_info, split_seg, n_split, part,
-                               (float*) y, alpha, accumulate);
+            hipLaunchKernelGGL((k_csr_split_rows<float, false>), dim3(b), dim3(256), 0, s, seg_info, split_seg,
+                               n_split, part, (float*) y, alpha, accumulate, o);
     }
+    return (int) hipGetLastError();
+}
+
+template<typename TC>
+int
+csr_quadratic_t(const int64_t* indptr, const TC* col, const float* val, const float* x, const int64_t* seg_k,
+                const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+                double alpha, quad_obj o, double inv_n, double* f, hipStream_t s) {
+    if (n_seg <= 0 || !o.part || !o.p || !f) return (int) hipErrorInvalidValue;
+    o.n_seg          = n_seg;
+    const int blocks = (int) (n_seg < 262144 ? n_seg : 262144);
+    hipLaunchKernelGGL((k_csr_adaptive<double, TC, true>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k,
+                       seg_info, n_seg, part, (double*) nullptr, alpha, 0, o);
+    if (n_split > 0)
+        hipLaunchKernelGGL((k_csr_split_rows<double, true>), dim3((unsigned) ((n_split + 255) / 256)), dim3(256), 0, s,
+                           seg_info, split_seg, n_split, part, (double*) nullptr, alpha, 0, o);
+    hipLaunchKernelGGL(k_quad_sum, dim3(1), dim3(256), 0, s, o.part, n_seg + n_split, inv_n, f);
     return (int) hipGetLastError();
 }
 
@@ -513,6 +568,21 @@ csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, con
                void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
     return csr_adaptive_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, y, y_is_double,
                           alpha, accumulate, s);
+}
+
+int
+csr_quadratic(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
+              const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+              double alpha, quad_obj o, double inv_n, double* f, hipStream_t s) {
+    return csr_quadratic_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, alpha, o, inv_n, f,
+                           s);
+}
+int
+csr_quadratic16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
+                const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
+                double alpha, quad_obj o, double inv_n, double* f, hipStream_t s) {
+    return csr_quadratic_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, alpha, o, inv_n, f,
+                           s);
 }
 
 // ------------------------------------------------------------------ D^T g over column deltas

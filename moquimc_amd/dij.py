@@ -5,12 +5,14 @@ A Dij built by ``Context.dij_build`` is a CSR matrix with one row per spot
 moqui/base/mqi_io.hpp:249-320).  With D the voxels x spots influence
 matrix, the dose of spot weights w is D w and the gradient of an objective
 f(D w) with respect to w is D^T (df/dd).  Both products run in the
-library's CSR-Adaptive kernels (FP64 accumulation, float32 inputs) on
-torch's current stream; ``dose`` wraps them in a ``torch.autograd.Function``.
+library's kernels (FP64 accumulation, float32 inputs, fixed summation
+order) on torch's current stream; ``dose`` wraps them in a
+``torch.autograd.Function``, and ``quadratic_objective`` forms the
+quadratic objective and its dose gradient inside the D w pass.
 
     mat = DijMatrix(ctx, scenario=0)
     w = torch.rand(mat.n_spots, device="cuda", requires_grad=True)
-    f = quadratic_objective(mat, w, prescription)
+    f = quadratic_objective(mat, w, prescription)   # fused: one pass, no dose vector
     f.backward()                      # w.grad = 2 D^T (D w - p) / n
 
 Robust optimisation: ``worst_case_objective`` over the Dij of several
@@ -50,6 +52,26 @@ class DijMatrix:
         """Dose scale * D w (n_voxels)."""
         return self._product(False, self._vec(w, self.n_spots), self.n_voxels, out_dtype)
 
+    def quadratic(self, w, prescription, voxel_weights=None, want_grad=True):
+        """f = mean_v c_v (scale (D w)_v - p_v)^2 as a device scalar (float64)
+        and, with want_grad, df/dd = 2 c (d - p) / n_voxels (float32, the
+        input of rmatvec), in one pass over the matrix (mqi_dij_quadratic)."""
+        w = self._vec(w, self.n_spots)
+        p = prescription.detach().to(device=w.device, dtype=torch.float64).contiguous()
+        if p.dim() != 1 or p.numel() != self.n_voxels:
+            raise ValueError("expected a prescription of %d voxels, got shape %s" % (self.n_voxels, tuple(p.shape)))
+        c = None
+        if voxel_weights is not None:
+            c = voxel_weights.detach().to(device=w.device, dtype=torch.float32).contiguous()
+            if c.dim() != 1 or c.numel() != self.n_voxels:
+                raise ValueError("expected voxel weights of %d voxels" % self.n_voxels)
+        f = torch.empty((), dtype=torch.float64, device=w.device)
+        g = torch.empty(self.n_voxels, dtype=torch.float32, device=w.device) if want_grad else None
+        stream = torch.cuda.current_stream(w.device).cuda_stream
+        self.ctx.dij_quadratic(self.scenario, w.data_ptr(), p.data_ptr(), c.data_ptr() if c is not None else 0,
+                               self.scale, g.data_ptr() if g is not None else 0, f.data_ptr(), stream=stream)
+        return f, g
+
     def rmatvec(self, g, out_dtype=torch.float64):
         """scale * D^T g (n_spots)."""
         return self._product(True, self._vec(g, self.n_voxels), self.n_spots, out_dtype)
@@ -73,13 +95,28 @@ def dose(mat, w):
     return _Dose.apply(w, mat)
 
 
+class _Quadratic(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, w, mat, prescription, voxel_weights):
+        f, g = mat.quadratic(w, prescription, voxel_weights, want_grad=fctx.needs_input_grad[0])
+        fctx.mat = mat
+        fctx.w_dtype = w.dtype
+        if g is not None:
+            fctx.save_for_backward(g)
+        return f
+
+    @staticmethod
+    def backward(fctx, grad_f):
+        (g,) = fctx.saved_tensors
+        grad_w = fctx.mat.rmatvec(g, torch.float64) * grad_f
+        return grad_w.to(fctx.w_dtype), None, None, None
+
+
 def quadratic_objective(mat, w, prescription, voxel_weights=None):
-    """f(w) = mean_v c_v (D w - p)_v^2 (c = 1 when voxel_weights is None)."""
-    d = dose(mat, w) - prescription.to(torch.float64)
-    r = d * d
-    if voxel_weights is not None:
-        r = r * voxel_weights.to(torch.float64)
-    return r.mean()
+    """f(w) = mean_v c_v (D w - p)_v^2 (c = 1 when voxel_weights is None), in
+    one pass over the matrix (the dose is not formed); the gradient is
+    D^T (2 c (D w - p) / n)."""
+    return _Quadratic.apply(w, mat, prescription, voxel_weights)
 
 
 def worst_case_objective(mats, w, prescription, voxel_weights=None):
@@ -91,6 +128,6 @@ def worst_case_objective(mats, w, prescription, voxel_weights=None):
     would carry zero gradients through the others).  The products are
     deterministic: the returned maximum equals its entry in the values."""
     with torch.no_grad():
-        vals = torch.stack([quadratic_objective(m, w, prescription, voxel_weights) for m in mats])
+        vals = torch.stack([m.quadratic(w, prescription, voxel_weights, want_grad=False)[0] for m in mats])
     k = int(torch.argmax(vals).item())
     return quadratic_objective(mats[k], w, prescription, voxel_weights), vals

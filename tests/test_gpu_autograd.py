@@ -105,3 +105,39 @@ def test_worst_case_over_scenarios(ctx):
     assert np.allclose(vals.detach().cpu().numpy(), [v for v, _ in ref], rtol=1e-9, atol=0)
     g_ref = ref[k][1]
     assert np.abs(w.grad.detach().cpu().numpy() - g_ref).max() <= 1e-5 * np.abs(g_ref).max()
+
+
+def test_fused_quadratic_with_voxel_weights_and_dose_path(ctx):
+    """The fused objective (one pass over D w, no dose vector) with voxel
+    weights against scipy in FP64 and against the objective formed from the
+    differentiable dose; without a graph it forms no gradient buffer."""
+    import torch
+    from moquimc_amd import dij
+    g, sp = _plan(ctx, scenarios=False)
+    mat = dij.DijMatrix(ctx, 0, scale=0.75)
+    D = _scipy(ctx, 0) * 0.75
+    rng = np.random.default_rng(8)
+    w_np = rng.uniform(0.5, 1.5, mat.n_spots).astype(np.float32)
+    p_np = _prescription(g)
+    c_np = rng.uniform(0.0, 2.0, D.shape[0]).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    w = torch.tensor(w_np, device=dev, requires_grad=True)
+    p, c = torch.tensor(p_np, device=dev), torch.tensor(c_np, device=dev)
+    f = dij.quadratic_objective(mat, w, p, voxel_weights=c)
+    f.backward()
+    r = D @ w_np.astype(np.float64) - p_np
+    f_ref = np.mean(c_np * r * r)
+    g_ref = 2.0 * (D.T @ (c_np * r)) / D.shape[0]
+    assert abs(f.item() - f_ref) <= 1e-9 * abs(f_ref)
+    got = w.grad.detach().cpu().numpy().astype(np.float64)
+    assert np.abs(got - g_ref).max() <= 1e-5 * np.abs(g_ref).max()
+    # the same objective through the differentiable dose
+    w2 = torch.tensor(w_np, device=dev, requires_grad=True)
+    d = dij.dose(mat, w2) - p.double()
+    f2 = (c.double() * d * d).mean()
+    f2.backward()
+    assert abs(f.item() - f2.item()) <= 1e-12 * abs(f2.item())
+    assert torch.allclose(w.grad.double(), w2.grad.double(), rtol=1e-6, atol=1e-9 * float(np.abs(g_ref).max()))
+    with torch.no_grad():
+        f3, g3 = mat.quadratic(w, p, c, want_grad=False)
+    assert g3 is None and f3.item() == f.item()
