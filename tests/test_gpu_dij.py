@@ -301,16 +301,17 @@ def _check_products(ctx, D, scen, rng):
     return got_grad, got_dose
 
 
-@pytest.mark.parametrize("kind", ["long", "short", "many_rows"])
+@pytest.mark.parametrize("kind", ["long", "short", "many_rows", "dense_voxels"])
 def test_set_csr_products_match_fp64(ctx, kind, tmp_path):
-    """A caller's CSR through mqi_dij_set_csr: both products against scipy
-    in FP64 on matrices built to reach every path of the kernels (long rows:
-    the byte-delta pieces with escapes, escape reloads and 65535-column
-    steps; short rows: the compacted CSR-Adaptive form; more than 65536 rows:
-    32-bit spot columns in D w), the matrix read back unchanged, and the
-    same matrix through the .npz writer and loader giving the same results
-    bit for bit."""
-    rng = np.random.default_rng({"long": 1, "short": 2, "many_rows": 3}[kind])
+    """A caller's CSR through mqi_dij_set_csr: both products and the fused
+    quadratic objective against scipy in FP64 on matrices built to reach
+    every path of the kernels (long rows: the byte-delta pieces with
+    escapes, escape reloads and 65535-column steps; short rows: the
+    compacted CSR-Adaptive form; more than 65536 rows: 32-bit spot columns
+    in D w; voxels with more than 1024 spots: the split rows of D w), the
+    matrix read back unchanged, and the same matrix through the .npz writer
+    and loader giving the same results bit for bit."""
+    rng = np.random.default_rng({"long": 1, "short": 2, "many_rows": 3, "dense_voxels": 6}[kind])
     if kind == "long":
         rows, cols = _adversarial_long(rng)
         D = _to_csr(rows, cols, rng)
@@ -318,9 +319,13 @@ def test_set_csr_products_match_fp64(ctx, kind, tmp_path):
     elif kind == "short":
         D = sps.random(3000, 100_000, density=2e-4, format="csr", dtype=np.float32, random_state=4)
         D.data = rng.uniform(1e-6, 1.0, D.nnz).astype(np.float32)
-    else:
+    elif kind == "many_rows":
         D = sps.random(70_000, 5000, density=4e-4, format="csr", dtype=np.float32, random_state=5)
         D.data = rng.uniform(1e-6, 1.0, D.nnz).astype(np.float32)
+    else:
+        D = sps.random(3000, 60, density=0.8, format="csr", dtype=np.float32, random_state=7)
+        D.data = rng.uniform(1e-6, 1.0, D.nnz).astype(np.float32)
+        assert np.diff(D.tocsc().indptr).max() > 1024
     D.sort_indices()
     ctx.dij_set_csr(D.indptr, D.indices, D.data, D.shape, scenario=0)
     assert ctx.dij_info(0) == (D.nnz, D.shape[0], D.shape[1])
@@ -333,6 +338,39 @@ def test_set_csr_products_match_fp64(ctx, kind, tmp_path):
     ctx.dij_load_npz(path, scenario=1)
     b = _check_products(ctx, D, 1, np.random.default_rng(9))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # the fused objective and its dose gradient
+    import torch
+    from moquimc_amd import dij
+    dev = torch.device("cuda", 0)
+    w_np = rng.uniform(0.5, 1.5, D.shape[0]).astype(np.float32)
+    p_np = rng.uniform(0.0, 2.0, D.shape[1]) * float(np.abs(D.T @ w_np.astype(np.float64)).mean())
+    c_np = rng.uniform(0.0, 2.0, D.shape[1]).astype(np.float32)
+    mat = dij.DijMatrix(ctx, 0, scale=1.5)
+    f, g = mat.quadratic(torch.from_numpy(w_np).to(dev), torch.from_numpy(p_np).to(dev),
+                         torch.from_numpy(c_np).to(dev), want_grad=True)
+    r = 1.5 * (D.T.astype(np.float64) @ w_np.astype(np.float64)) - p_np
+    f_ref = np.mean(c_np * r * r)
+    g_ref = 2.0 * c_np * r / D.shape[1]
+    assert abs(f.item() - f_ref) <= 1e-9 * abs(f_ref)
+    assert np.allclose(g.cpu().numpy(), g_ref, rtol=2e-6, atol=1e-6 * np.abs(g_ref).max())
+
+
+def test_quadratic_rejects_bad_input(ctx):
+    """The fused objective: a scenario without a Dij is a state error and a
+    prescription of the wrong length an argument error."""
+    import torch
+    from moquimc_amd import dij
+    from moquimc_amd._abi import MqiError
+    D = sps.random(20, 300, density=0.3, format="csr", dtype=np.float32, random_state=1)
+    ctx.dij_set_csr(D.indptr, D.indices, D.data, D.shape, scenario=0)
+    dev = torch.device("cuda", 0)
+    w = torch.ones(20, device=dev)
+    f = torch.empty((), dtype=torch.float64, device=dev)
+    p = torch.zeros(300, dtype=torch.float64, device=dev)
+    with pytest.raises(MqiError, match="no Dij"):
+        ctx.dij_quadratic(5, w.data_ptr(), p.data_ptr(), 0, 1.0, 0, f.data_ptr())
+    with pytest.raises(ValueError):
+        dij.DijMatrix(ctx, 0).quadratic(w, torch.zeros(299, device=dev))
 
 
 def test_set_csr_rejects_malformed(ctx):
