@@ -310,6 +310,20 @@ def transport_pass(waves=3, block=128, fetch=2, refill=8):
                                  "refill_lanes": refill}}), flush=True)
 
 
+def atomic_probes(n_slots=2 * 512 * 512 * 200, n_updates=1 << 30):
+    """FP64 atomic adds per second into two CT-dose-sized arrays, uniform and
+    with each wave's 64 adds inside windows of 64 .. 262144 slots: how
+    address clustering raises the rate the transport's scoring runs at."""
+    from moquimc_amd import _abi
+    ctx = _abi.Context(0)
+    out = {}
+    for window in (0, 64, 512, 4096, 32768, 262144):
+        ms = ctx.atomic_probe(n_slots, n_updates, window)
+        out["uniform" if window == 0 else "window_%d" % window] = float("%.4g" % (n_updates / (ms * 1e-3)))
+    ctx.close()
+    print(json.dumps({"atomics_per_s": out, "n_slots": n_slots, "n_updates": n_updates}), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=3):
     """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its
     dispatches against nbytes give the scale of the HBM counters."""
@@ -405,6 +419,8 @@ def main():
     ap.add_argument("--phantoms-only", action="store_true", help="only the water and slab phantom leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the reference kernel of the HBM counter runs)")
+    ap.add_argument("--atomic-probes", action="store_true",
+                    help="only the FP64 atomic rate, uniform and clustered (mqi_atomic_probe)")
     ap.add_argument("--transport-pass", action="store_true",
                     help="only one production pass of the CT plan with a fixed launch shape (the counter runs)")
     args = ap.parse_args()
@@ -418,6 +434,9 @@ def main():
         return
     if args.transport_pass:
         transport_pass()
+        return
+    if args.atomic_probes:
+        atomic_probes()
         return
     if args.robust_only:
         from moquimc_amd import _abi

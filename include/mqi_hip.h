@@ -318,8 +318,11 @@ int mqi_physics_probe(mqi_ctx* ctx, int exact, int physics_debug, int fn, const 
  * FP64 atomic adds to uniformly hashed slots of an n_slots array
  * (1 <= n_slots <= 2^32), the transport's scoring stream without the rest of
  * the step; *ms is the device time of one pass after a warm-up pass.  The
- * throughput ceiling the transport's scoring atomics are measured against. */
-int mqi_atomic_probe(mqi_ctx* ctx, int64_t n_slots, int64_t n_updates, double* ms);
+ * throughput ceiling the transport's scoring atomics are measured against.
+ * window > 0 (a power of two below n_slots): each wave's 64 adds fall in one
+ * window of that many slots placed at random, to measure how clustering
+ * raises the rate; 0: uniform. */
+int mqi_atomic_probe(mqi_ctx* ctx, int64_t n_slots, int64_t n_updates, int64_t window, double* ms);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

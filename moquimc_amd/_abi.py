@@ -74,7 +74,7 @@ def lib():
             "mqi_run": (i, [vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
             "mqi_run_vertices": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
             "mqi_run_vertices_noalloc": (i, [vp, vp, u64, vp, ctypes.POINTER(RunConfig), ctypes.POINTER(RunResult)]),
-            "mqi_atomic_probe": (i, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(d)]),
+            "mqi_atomic_probe": (i, [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(d)]),
             "mqi_trace": (i, [vp, ctypes.POINTER(RunConfig), vp, u64, i, vp, vp, vp]),
             "mqi_get_scorer": (i, [vp, i, i, vp]),
             "mqi_scorer_device_ptr": (i, [vp, i, i, ctypes.POINTER(vp)]),
@@ -424,11 +424,12 @@ class Context:
         self._chk(self._L.mqi_math_probe(self.h, fn, _p(x), _p(y), len(x), _p(out)), "mqi_math_probe")
         return out
 
-    def atomic_probe(self, n_slots, n_updates):
+    def atomic_probe(self, n_slots, n_updates, window=0):
         """Device ms of n_updates FP64 atomic adds to hashed slots of an
-        n_slots array (mqi_atomic_probe)."""
+        n_slots array, each wave's adds inside one window of that many slots
+        when window > 0 (mqi_atomic_probe)."""
         ms = ctypes.c_double()
-        self._chk(self._L.mqi_atomic_probe(self.h, n_slots, n_updates, ctypes.byref(ms)), "mqi_atomic_probe")
+        self._chk(self._L.mqi_atomic_probe(self.h, n_slots, n_updates, window, ctypes.byref(ms)), "mqi_atomic_probe")
         return ms.value
 
     def physics_probe(self, fn, rho, ek=None, exact=True, physics_debug=False):

@@ -2026,17 +2026,19 @@ mqi_math_probe(mqi_ctx* c, int fn, const double* x, const double* y, int64_t n, 
 }
 
 int
-mqi_atomic_probe(mqi_ctx* c, int64_t n_slots, int64_t n_updates, double* ms) {
+mqi_atomic_probe(mqi_ctx* c, int64_t n_slots, int64_t n_updates, int64_t window, double* ms) {
     if (!c || !ms || n_slots <= 0 || n_updates <= 0 || n_slots > (int64_t(1) << 32)) return MQI_E_ARG;
+    if (window < 0 || (window > 0 && (window >= n_slots || (window & (window - 1)) != 0))) return MQI_E_ARG;
     hipSetDevice(c->device);
     double* d = nullptr;
     int     r = dalloc(c, &d, (size_t) n_slots);
     if (r) return r;
     int e = mqi_aux::fill_u64(d, 0ull, (uint64_t) n_slots, c->stream);
-    if (!e) e = mqi_aux::atomic_probe(d, (uint64_t) n_slots, (uint64_t) n_updates, c->stream);   // warm-up
+    // a warm-up pass, then the timed one
+    if (!e) e = mqi_aux::atomic_probe(d, (uint64_t) n_slots, (uint64_t) n_updates, (uint64_t) window, c->stream);
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
     if (!e) e = (int) hipEventRecord(e0, c->stream);
-    if (!e) e = mqi_aux::atomic_probe(d, (uint64_t) n_slots, (uint64_t) n_updates, c->stream);
+    if (!e) e = mqi_aux::atomic_probe(d, (uint64_t) n_slots, (uint64_t) n_updates, (uint64_t) window, c->stream);
     if (!e) e = (int) hipEventRecord(e1, c->stream);
     if (!e) e = (int) hipEventSynchronize(e1);
     float t = 0;

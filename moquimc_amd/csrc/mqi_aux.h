@@ -13,8 +13,10 @@ int queue_init(unsigned long long* q, uint64_t total, hipStream_t s);
 int fill_u32(void* p, uint32_t v, uint64_t n, hipStream_t s);
 int hu_to_density(const int16_t* hu, float* rho, uint64_t n, float scale, hipStream_t s);
 int math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hipStream_t s);
-// n_updates FP64 atomic adds to hashed slots of a (n_slots doubles)
-int atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, hipStream_t s);
+// n_updates FP64 atomic adds to hashed slots of a (n_slots doubles); with
+// window > 0 (a power of two below n_slots) the 64 adds of each wave fall in
+// one window of that many slots placed at random
+int atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, uint64_t window, hipStream_t s);
 int stat_reduce(const double* s1, const double* s2, uint64_t n, double N, float thr, float* d_max, double* d_sum,
                 unsigned long long* d_cnt, hipStream_t s);
 int scale(double* p, uint64_t n, double w, hipStream_t s);

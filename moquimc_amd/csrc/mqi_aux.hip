@@ -108,27 +108,39 @@ math_probe(int fn, const double* x, const double* y, double* out, int64_t n, hip
 // update index over n_slots: the scoring pattern of the transport (random
 // voxels, device-scope adds whose results are not read) without the rest of
 // the step, to measure the part's throughput for such atomics.
-__global__ void k_atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates) {
-    // slot = the murmur3 32-bit finaliser of i scaled to [0, n_slots) by a
-    // high multiply (n_slots <= 2^32): a few integer operations per update,
-    // so the probe measures the atomic stream, not a 64-bit modulo
+MQI_DI uint32_t fmix32(uint32_t h) {   // the murmur3 32-bit finaliser
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// slot = a hash of i scaled to [0, n_slots) by a high multiply (n_slots <=
+// 2^32): a few integer operations per update, so the probe measures the
+// atomic stream, not a 64-bit modulo.  window > 0: the wave's 64 updates
+// (i with the same i / 64) share one window [base, base + window).
+__global__ void k_atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, uint64_t window) {
     for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n_updates;
          i += (uint64_t) gridDim.x * blockDim.x) {
-        uint32_t h = (uint32_t) i;
-        h ^= h >> 16;
-        h *= 0x85EBCA6Bu;
-        h ^= h >> 13;
-        h *= 0xC2B2AE35u;
-        h ^= h >> 16;
-        atomicAdd(a + (((uint64_t) h * n_slots) >> 32), 1.0);
+        uint64_t slot;
+        if (window == 0) {
+            slot = ((uint64_t) fmix32((uint32_t) i) * n_slots) >> 32;
+        } else {
+            const uint64_t base = ((uint64_t) fmix32((uint32_t) (i >> 6) ^ 0x9E3779B9u) * (n_slots - window)) >> 32;
+            slot                = base + (fmix32((uint32_t) i) & (uint32_t) (window - 1));
+        }
+        atomicAdd(a + slot, 1.0);
     }
 }
 
 int
-atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, hipStream_t s) {
+atomic_probe(double* a, uint64_t n_slots, uint64_t n_updates, uint64_t window, hipStream_t s) {
     if (n_slots == 0 || n_updates == 0) return 0;
     if (n_slots > (1ull << 32)) return (int) hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_atomic_probe, dim3(65536), dim3(256), 0, s, a, n_slots, n_updates);
+    if (window != 0 && (window >= n_slots || (window & (window - 1)) != 0)) return (int) hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_atomic_probe, dim3(65536), dim3(256), 0, s, a, n_slots, n_updates, window);
     return (int) hipGetLastError();
 }
 

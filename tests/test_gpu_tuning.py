@@ -126,6 +126,8 @@ def test_atomic_probe_times_and_rejects_bad_sizes(ctx):
     ms = ctx.atomic_probe(1 << 20, 1 << 22)
     assert ms > 0.0
     assert (1 << 22) / (ms * 1e-3) > 1e8    # at least 1e8 atomics/s
-    for n_slots, n_updates in [((1 << 32) + 1, 1 << 10), (0, 1 << 10), (1 << 10, 0)]:
+    assert ctx.atomic_probe(1 << 20, 1 << 22, window=64) > 0.0
+    for n_slots, n_updates, window in [((1 << 32) + 1, 1 << 10, 0), (0, 1 << 10, 0), (1 << 10, 0, 0),
+                                       (1 << 10, 1 << 10, 48), (1 << 10, 1 << 10, 1 << 10), (1 << 10, 1 << 10, -1)]:
         with pytest.raises(RuntimeError):
-            ctx.atomic_probe(n_slots, n_updates)
+            ctx.atomic_probe(n_slots, n_updates, window)
