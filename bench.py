@@ -75,7 +75,9 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers, atomic_ceiling):
         out["atomic_roofline"] = {"achieved": float("%.4g" % rate), "peak": atomic_ceiling, "unit": "atomics/s",
                                   "frac": round(rate / atomic_ceiling, 4) if atomic_ceiling else None,
                                   "atomics_per_voxel_step": round(m["atomics"] / m["voxel_steps"], 4),
-                                  "peak_source": "mqi_atomic_probe, random FP64 atomic adds on this device, this run",
+                                  "peak_source": ("mqi_atomic_probe, uniformly random FP64 atomic adds on this "
+                                                  "device, this run (clustered streams run faster: "
+                                                  "profiles/atomic_probes.md)"),
                                   "counter_source": COUNTER_SOURCE}
     out["limiter"] = ("the FP64 atomic stream of the two scored arrays (atomic_roofline); VALU issue and HBM "
                       "bandwidth are not at their ceilings (" + COUNTER_SOURCE + ")")
