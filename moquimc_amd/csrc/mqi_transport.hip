@@ -514,17 +514,18 @@ MQI_DI void dij_insert(const transport_params& P, uint64_t item, uint32_t voxel,
     atomicOr(P.status, MQI_STATUS_DIJ_FULL);
 }
 
-// Deposits of consecutive steps into the same voxel (and scenario) are
-// summed per lane and added to the scorer arrays once, when the lane moves
-// on (each-scorer-once scoring only; the sums differ from per-step atomics
-// only in FP64 summation order).  Scorer slots beyond MQI_DEFER are scored
-// per step.  The sums live in the lane's LDS slots (v[s * blockDim.x]), not
+// Deposits of consecutive steps into the same voxel (and scenario, and Dij
+// table) are summed per lane and added to the scorer arrays, or inserted
+// into the Dij table, once, when the lane moves on (each-scorer-once scoring
+// only; the sums differ from per-step atomics only in FP64 summation order).
+// Scorer slots beyond MQI_DEFER are scored per step.  The sums live in the lane's LDS slots (v[s * blockDim.x]), not
 // in registers: held across the loop they pushed the kernel into scratch,
 // and every scratch reload waits on the one counter the atomics in flight
 // also hold.
 struct pending {
-    int64_t  cnb;   // voxel of the pending sums, -1: none
-    uint64_t off;   // scenario offset of the pending sums
+    int64_t  cnb;    // voxel of the pending sums, -1: none
+    uint64_t off;    // scenario offset of the pending sums
+    uint32_t item;   // Dij table (scenario, spot) of the pending sums
 };
 
 MQI_DI void flush(const transport_params& P, pending& A, double* v) {
@@ -534,7 +535,11 @@ MQI_DI void flush(const transport_params& P, pending& A, double* v) {
 #pragma unroll
         for (int s = 0; s < MQI_DEFER; ++s) {
             const double x = v[s * blockDim.x];
-            if (x != 0.0) atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, x);
+            if (x == 0.0) continue;
+            if (P.kinds[s] == MQI_KIND_DIJ)
+                dij_insert(P, A.item, (uint32_t) A.cnb, x);
+            else
+                atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, x);
         }
     }
     A.cnb = -1;
@@ -577,10 +582,11 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             v[s] = (k == MQI_KIND_STAT_DOSE2) ? d * d : d;
     }
     if (P.loops == MQI_LOOPS_ONCE) {
-        if ((int64_t) cnb != A.cnb || scen_off != A.off) {
+        if ((int64_t) cnb != A.cnb || scen_off != A.off || (uint32_t) dij_item != A.item) {
             flush(P, A, pv);
-            A.cnb = (int64_t) cnb;
-            A.off = scen_off;
+            A.cnb  = (int64_t) cnb;
+            A.off  = scen_off;
+            A.item = (uint32_t) dij_item;
 #pragma unroll
             for (int s = 0; s < MQI_DEFER; ++s) pv[s * blockDim.x] = 0.0;
         }
@@ -590,10 +596,10 @@ MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float d
             const int  k    = P.kinds[s];
             const bool stat = (k == MQI_KIND_STAT_DOSE || k == MQI_KIND_STAT_DOSE2);
             if (!(stat ? in_stat : in_reg) || v[s] <= 0) continue;
-            if (k == MQI_KIND_DIJ)
-                dij_insert(P, dij_item, (uint32_t) cnb, v[s]);
-            else if (s < MQI_DEFER)
+            if (s < MQI_DEFER)
                 pv[s * blockDim.x] += v[s];
+            else if (k == MQI_KIND_DIJ)
+                dij_insert(P, dij_item, (uint32_t) cnb, v[s]);
             else
                 atomicAdd(P.out[s] + scen_off + cnb, v[s]);
         }
@@ -873,7 +879,8 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb, d
         // step: one counter tracks loads, stores and atomics alike, so atomics
         // issued just before the next wait on a load would add their round
         // trip to every step.
-        if (P.loops == MQI_LOOPS_ONCE && ((int64_t) cnb != L.acc.cnb || L.scen_off != L.acc.off))
+        if (P.loops == MQI_LOOPS_ONCE &&
+            ((int64_t) cnb != L.acc.cnb || L.scen_off != L.acc.off || (uint32_t) L.dij_item != L.acc.item))
             flush(P, L.acc, pv);
     }
     float sp;   // stopping power ratio of the step
@@ -1005,8 +1012,10 @@ k_transport(transport_params P) {
     L.nsteps   = 0;
     L.scale    = 1.0f;
     L.scen_off = 0;
+    L.dij_item = 0;
     L.acc.cnb  = -1;
     L.acc.off  = 0;
+    L.acc.item = 0;
     L.reenter  = false;
     c.g.sx = c.g.sy = c.g.sz = 0.0f;
     unsigned long long done = 0, nstep = 0;
