@@ -503,9 +503,14 @@ MQI_DI void dij_insert(const transport_params& P, uint64_t item, uint32_t voxel,
     uint32_t       h    = voxel * 2654435761u;
     uint64_t       slot = ((uint64_t) h * cap) >> 32;
     const uint64_t n_probe = cap < MQI_DIJ_MAX_PROBE ? cap : MQI_DIJ_MAX_PROBE;
+    // A key changes once, from empty to its voxel, so a plain load that
+    // shows a voxel is final: the slot is ours, or taken by another voxel.
+    // Only a load that shows it empty (possibly a stale cached line) needs
+    // the compare-and-swap, which is performed beyond the caches.
     for (uint64_t probe = 0; probe < n_probe; ++probe) {
-        uint32_t prev = atomicCAS(keys + slot, 0xffffffffu, voxel);
-        if (prev == 0xffffffffu || prev == voxel) {
+        uint32_t k = keys[slot];
+        if (k == 0xffffffffu) k = atomicCAS(keys + slot, 0xffffffffu, voxel);
+        if (k == 0xffffffffu || k == voxel) {
             atomicAdd(vals + slot, value);
             return;
         }
