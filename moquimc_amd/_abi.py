@@ -128,6 +128,27 @@ class MqiError(RuntimeError):
     pass
 
 
+def csr_canonical(indptr, indices, data):
+    """The CSR with each row's columns ascending and repeated columns summed
+    (in float64, then float32): the reference's io::save_to_npz writes each
+    row in hash-table order (mqi_io.hpp:249-320), and mqi_dij_set_csr takes
+    rows with strictly ascending columns."""
+    indptr = np.asarray(indptr, np.int64)
+    indices = np.asarray(indices, np.int64)[:indptr[-1]]
+    data = np.asarray(data, np.float64)[:indptr[-1]]
+    rows = np.repeat(np.arange(indptr.size - 1, dtype=np.int64), np.diff(indptr))
+    order = np.lexsort((indices, rows))
+    rows, indices, data = rows[order], indices[order], data[order]
+    if indices.size:
+        first = np.r_[True, (rows[1:] != rows[:-1]) | (indices[1:] != indices[:-1])]
+        starts = np.flatnonzero(first)
+        data = np.add.reduceat(data, starts)
+        rows, indices = rows[starts], indices[starts]
+    out_ptr = np.zeros(indptr.size, np.int64)
+    np.add.at(out_ptr, rows + 1, 1)
+    return np.cumsum(out_ptr), indices.astype(np.int32), data.astype(np.float32)
+
+
 def _p(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
@@ -369,15 +390,16 @@ class Context:
     def dij_load_npz(self, path, scenario=0):
         """A scipy-style CSR .npz (the layout io::save_to_npz and
         mqi_dij_save_npz write: rows = spots, columns = voxels) as the Dij of
-        scenario; values as float32."""
+        scenario: rows put in column order (csr_canonical), values as
+        float32."""
         with np.load(path, allow_pickle=False) as z:
             fmt = z["format"][()]
             fmt = fmt.decode() if isinstance(fmt, bytes) else str(fmt)
             if fmt != "csr":
                 raise ValueError("%s is not a CSR matrix (format %r)" % (path, fmt))
             shape = [int(v) for v in z["shape"]]
-            self.dij_set_csr(z["indptr"].astype(np.int64), z["indices"].astype(np.int32),
-                             z["data"].astype(np.float32), shape, scenario)
+            ip, ix, v = csr_canonical(z["indptr"], z["indices"], z["data"])
+            self.dij_set_csr(ip, ix, v, shape, scenario)
 
     def dij_csr(self, scenario=0):
         nnz, rows, cols = self.dij_info(scenario)

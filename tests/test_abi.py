@@ -222,3 +222,25 @@ def test_dij_size_guard_on_host(tmp_path):
                     str(src), "-o", exe], check=True)
     got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert got == [0, 0, 0, 0, 5, 5, 5]   # MQI_OK ... MQI_E_CAPACITY
+
+
+def test_csr_canonical_sorts_rows_and_sums_repeats():
+    """A CSR in the reference's hash-table row order, with repeated columns
+    and an empty row, becomes the canonical CSR scipy makes of it (columns
+    ascending per row, repeats summed)."""
+    import scipy.sparse as sps
+    from moquimc_amd import _abi
+    rng = np.random.default_rng(3)
+    rows, cols = 40, 500
+    lens = rng.integers(0, 30, rows)
+    lens[7] = 0
+    indptr = np.r_[0, np.cumsum(lens)].astype(np.uint32)
+    indices = rng.integers(0, cols, int(indptr[-1])).astype(np.uint32)   # unsorted, with repeats
+    data = rng.uniform(1e-6, 1.0, indices.size)
+    ip, ix, v = _abi.csr_canonical(indptr, indices, data)
+    want = sps.csr_matrix((data, indices.astype(np.int64), indptr.astype(np.int64)), shape=(rows, cols))
+    want.sum_duplicates()
+    want.sort_indices()
+    assert np.array_equal(ip, want.indptr) and np.array_equal(ix, want.indices)
+    assert np.array_equal(v, want.data.astype(np.float32))
+    assert all(np.all(np.diff(ix[ip[r]:ip[r + 1]]) > 0) for r in range(rows))

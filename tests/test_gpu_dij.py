@@ -384,3 +384,24 @@ def test_set_csr_rejects_malformed(ctx):
                           ([0, 1, 2, 3], np.array([0, 3, 2], np.int64), "indptr")]:
         with pytest.raises(MqiError, match=msg):
             ctx.dij_set_csr(ipp, np.array(idx, np.int32), v, (2, 10))
+
+
+def test_load_npz_in_reference_row_order(ctx, tmp_path):
+    """A Dij .npz laid out as the reference writes it (rows in hash-table
+    order, uint32 indices and pointers, float64 values) loads as the
+    canonical matrix: both products equal scipy's FP64 products of it."""
+    rng = np.random.default_rng(12)
+    rows, cols = 30, 20000
+    lens = rng.integers(0, 3000, rows)
+    indptr = np.r_[0, np.cumsum(lens)].astype(np.uint32)
+    indices = np.concatenate([rng.permutation(rng.choice(cols, n, replace=False)) for n in lens]).astype(np.uint32)
+    data = rng.uniform(1e-6, 1.0, indices.size)
+    path = str(tmp_path / "ref.npz")
+    np.savez(path, indices=indices, indptr=indptr, shape=np.array([rows, cols], np.uint32), data=data,
+             format=np.array("csr"))
+    ctx.dij_load_npz(path, scenario=0)
+    D = sps.csr_matrix((data.astype(np.float32), indices.astype(np.int64), indptr.astype(np.int64)),
+                       shape=(rows, cols))
+    D.sort_indices()
+    assert ctx.dij_info(0) == (D.nnz, rows, cols)
+    _check_products(ctx, D, 0, np.random.default_rng(13))
