@@ -34,7 +34,7 @@ struct csr_plan {
     uint32_t* seg_info  = nullptr;   // n_seg + 1
     int32_t*  split_seg = nullptr;   // 2 per split row
     double*   part      = nullptr;   // n_seg
-    double*   qpart     = nullptr;   // n_seg + n_split: terms of the quadratic objective (allocated on first use)
+    double*   qpart     = nullptr;   // n_seg + n_split + MQI_QUAD_EMPTY_BLOCKS: objective terms (on first use)
 };
 
 struct csr_dev {
@@ -42,7 +42,8 @@ struct csr_dev {
     int64_t* indptr  = nullptr;   // rows + 1
     int32_t* col     = nullptr;
     float*   val     = nullptr;
-    int64_t* tindptr = nullptr;   // transposed (voxel-major) copy: cols + 1
+    int64_t* tindptr = nullptr;   // transposed (voxel-major) copy without its empty rows: n_nz + 1
+    uint32_t* tnonempty = nullptr;   // bit v: voxel v has a nonzero ((cols + 31) / 32 words)
     int32_t* tcol    = nullptr;
     float*   tval    = nullptr;
     csr_plan plan, tplan;         // segmentations of the spot-major and voxel-major forms
@@ -259,6 +260,7 @@ free_csr(csr_dev& m) {
         dfree(m.col);
         dfree(m.val);
         dfree(m.tindptr);
+        dfree(m.tnonempty);
         dfree(m.tcol);
         dfree(m.tval);
         dfree(m.nzvox);
@@ -1582,27 +1584,32 @@ prepare_products(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& indptr, boo
             hipFree(tv);
         }
         // voxel-major copy for D w: row lengths by counting, entries by a
-        // stable sort on the voxel (each voxel's spots ascending)
+        // stable sort on the voxel (each voxel's spots ascending); its rows
+        // are the voxels with a nonzero only (D w zeroes the others)
         uint32_t* d_cc = nullptr;
-        if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &m.tindptr, m.cols + 1)) ||
-            (r = dalloc(c, &m.tcol, m.nnz)) || (r = dalloc(c, &m.tval, m.nnz)))
+        if ((r = dalloc(c, &d_cc, m.cols)) || (r = dalloc(c, &m.tcol, m.nnz)) || (r = dalloc(c, &m.tval, m.nnz)))
             return r;
         KCHK(c, mqi_aux::fill_u32(d_cc, 0u, m.cols, c->stream));
         KCHK(c, mqi_aux::csr_transpose_count(m.col, m.nnz, d_cc, c->stream));
         std::vector<uint32_t> cc(m.cols);
         HIPCHK(c, hipMemcpy(cc.data(), d_cc, sizeof(uint32_t) * m.cols, hipMemcpyDeviceToHost));
-        std::vector<int64_t> tptr(m.cols + 1, 0);
-        for (int64_t v = 0; v < m.cols; ++v) tptr[v + 1] = tptr[v] + cc[v];
-        HIPCHK(c, hipMemcpy(m.tindptr, tptr.data(), sizeof(int64_t) * (m.cols + 1), hipMemcpyHostToDevice));
-        if ((r = build_plan(c, tptr, m.tplan))) return r;
-        // compacted gather of D^T g: voxels with a nonzero, and each one's rank
-        std::vector<int32_t> nz, rank(m.cols, 0);
+        // voxels with a nonzero (the rows of the voxel-major copy and the
+        // compacted gather of D^T g), each one's rank, and their bitmask
+        std::vector<int32_t>  nz, rank(m.cols, 0);
+        std::vector<int64_t>  tptr(1, 0);
+        std::vector<uint32_t> bits((m.cols + 31) / 32, 0u);
         for (int64_t v = 0; v < m.cols; ++v)
             if (cc[v]) {
                 rank[v] = (int32_t) nz.size();
                 nz.push_back((int32_t) v);
+                tptr.push_back(tptr.back() + cc[v]);
+                bits[v >> 5] |= 1u << (v & 31);
             }
-        m.n_nz       = (int64_t) nz.size();
+        m.n_nz = (int64_t) nz.size();
+        if ((r = dalloc(c, &m.tindptr, m.n_nz + 1)) || (r = dalloc(c, &m.tnonempty, bits.size()))) return r;
+        HIPCHK(c, hipMemcpy(m.tindptr, tptr.data(), sizeof(int64_t) * (m.n_nz + 1), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(m.tnonempty, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice));
+        if ((r = build_plan(c, tptr, m.tplan))) return r;
         int32_t* d_rank = nullptr;
         if ((r = dalloc(c, &m.nzvox, m.n_nz)) || (r = dalloc(c, &m.ccol, m.nnz)) || (r = dalloc(c, &m.xc, m.n_nz)) ||
             (r = dalloc(c, &d_rank, m.cols)))
@@ -1938,7 +1945,8 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
                 xp   = m.xc;
             }
             KCHK(c, mqi_aux::csr_adaptive(m.indptr, colp, m.val, xp, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
-                                          p.n_split, p.part, y, y_double, alpha, accumulate, st));
+                                          p.n_split, p.part, y, y_double, alpha, accumulate, nullptr, nullptr, 0,
+                                          st));
         }
     } else if (m.nnz == 0) {
         // D w of an empty matrix: zeros (or y unchanged)
@@ -1949,14 +1957,17 @@ mqi_dij_matvec(mqi_ctx* c, int s, int transpose, const float* x, void* y, int y_
                 KCHK(c, mqi_aux::fill_u32(y, 0u, m.cols, st));
         }
     } else {
-        // D w: rows of the voxel-major copy
+        // D w: rows of the voxel-major copy (the voxels with a nonzero),
+        // zeros at the others
         const csr_plan& p = m.tplan;
         if (m.tcol16)
             KCHK(c, mqi_aux::csr_adaptive16(m.tindptr, m.tcol16, m.tval, x, p.seg_k, p.seg_info, p.n_seg,
-                                            p.split_seg, p.n_split, p.part, y, y_double, alpha, accumulate, st));
+                                            p.split_seg, p.n_split, p.part, y, y_double, alpha, accumulate, m.nzvox,
+                                            m.tnonempty, m.cols, st));
         else
             KCHK(c, mqi_aux::csr_adaptive(m.tindptr, m.tcol, m.tval, x, p.seg_k, p.seg_info, p.n_seg, p.split_seg,
-                                          p.n_split, p.part, y, y_double, alpha, accumulate, st));
+                                          p.n_split, p.part, y, y_double, alpha, accumulate, m.nzvox, m.tnonempty,
+                                          m.cols, st));
     }
     HIPCHK(c, hipEventRecord(c->ev_mv1, st));
     c->matvec_pending = true;
@@ -1973,7 +1984,7 @@ mqi_dij_quadratic(mqi_ctx* c, int s, const float* w, const double* p, const floa
     if (m.nnz == 0) return fail(c, MQI_E_STATE, "scenario %d's Dij has no nonzeros", s);
     csr_plan& P = m.tplan;
     int       r = 0;
-    if (!P.qpart && (r = dalloc(c, &P.qpart, std::max<int64_t>(P.n_seg + P.n_split, 1)))) return r;
+    if (!P.qpart && (r = dalloc(c, &P.qpart, P.n_seg + P.n_split + MQI_QUAD_EMPTY_BLOCKS))) return r;
     hipStream_t       st = stream ? (hipStream_t) stream : c->stream;
     mqi_aux::quad_obj o;
     o.p         = p;
@@ -1986,10 +1997,10 @@ mqi_dij_quadratic(mqi_ctx* c, int s, const float* w, const double* p, const floa
     HIPCHK(c, hipEventRecord(c->ev_mv0, st));
     if (m.tcol16)
         KCHK(c, mqi_aux::csr_quadratic16(m.tindptr, m.tcol16, m.tval, w, P.seg_k, P.seg_info, P.n_seg, P.split_seg,
-                                         P.n_split, P.part, alpha, o, inv_n, f, st));
+                                         P.n_split, P.part, alpha, o, inv_n, f, m.nzvox, m.tnonempty, m.cols, st));
     else
         KCHK(c, mqi_aux::csr_quadratic(m.tindptr, m.tcol, m.tval, w, P.seg_k, P.seg_info, P.n_seg, P.split_seg,
-                                       P.n_split, P.part, alpha, o, inv_n, f, st));
+                                       P.n_split, P.part, alpha, o, inv_n, f, m.nzvox, m.tnonempty, m.cols, st));
     HIPCHK(c, hipEventRecord(c->ev_mv1, st));
     c->matvec_pending = true;
     return 0;

@@ -37,18 +37,29 @@ int segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tm
 #define MQI_SEG_SHORT 1024
 #endif
 #define MQI_SEG_ROWS 1024
+// rowmap: the element of y each row writes (NULL: row r writes y[r]).  A
+// matrix stored without its empty rows (rowmap set) writes y = 0 at the n_y
+// elements whose bit in nonempty is clear, from MQI_ZERO_BLOCKS workgroups of
+// the same launch (with accumulate it leaves them alone).
+#define MQI_ZERO_BLOCKS 1024
 int csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
                  const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-                 void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
+                 void* y, int y_is_double, double alpha, int accumulate, const int32_t* rowmap,
+                 const uint32_t* nonempty, int64_t n_y, hipStream_t s);
 // the same over 16-bit columns (D w of a Dij with at most 65536 spots)
 int csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
                    const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-                   void* y, int y_is_double, double alpha, int accumulate, hipStream_t s);
+                   void* y, int y_is_double, double alpha, int accumulate, const int32_t* rowmap,
+                   const uint32_t* nonempty, int64_t n_y, hipStream_t s);
 // The quadratic objective of y = alpha * A x on the D w matrix, without y:
 // f = inv_n * sum_r c_r (y_r - p_r)^2 (c NULL: 1) into *f (device), and
 // g_r = two_inv_n * c_r (y_r - p_r) (float32, when g is set).  part: n_seg +
 // n_split doubles of per-segment and per-split-row terms, summed in a fixed
-// order.
+// order.  With rowmap the matrix has no empty rows: the n_y elements whose
+// bit in nonempty is clear (d = 0) get their terms from a fixed grid of
+// MQI_QUAD_EMPTY_BLOCKS workgroups (part then holds n_seg + n_split +
+// MQI_QUAD_EMPTY_BLOCKS doubles).
+#define MQI_QUAD_EMPTY_BLOCKS 1024
 struct quad_obj {
     const double* p;
     const float*  c;
@@ -59,10 +70,12 @@ struct quad_obj {
 };
 int csr_quadratic(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
                   const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-                  double alpha, quad_obj o, double inv_n, double* f, hipStream_t s);
+                  double alpha, quad_obj o, double inv_n, double* f, const int32_t* rowmap, const uint32_t* nonempty,
+                  int64_t n_y, hipStream_t s);
 int csr_quadratic16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x,
                     const int64_t* seg_k, const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg,
-                    int64_t n_split, double* part, double alpha, quad_obj o, double inv_n, double* f, hipStream_t s);
+                    int64_t n_split, double* part, double alpha, quad_obj o, double inv_n, double* f,
+                    const int32_t* rowmap, const uint32_t* nonempty, int64_t n_y, hipStream_t s);
 // D^T g over column deltas (mqi_aux.hip): the 16-bit deltas of each row's
 // columns (0 at row starts and where a row skips more than 65535 columns,
 // those positions listed in gaps), the column before each piece's first

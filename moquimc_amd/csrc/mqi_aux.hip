@@ -368,12 +368,23 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
                                                      const int64_t* __restrict__ seg_k,
                                                      const uint32_t* __restrict__ seg_info, int64_t n_seg,
                                                      double* __restrict__ part, TY* __restrict__ y, double alpha,
-                                                     int accumulate, quad_obj o) {
+                                                     int accumulate, quad_obj o,
+                                                     const int32_t* __restrict__ rowmap,
+                                                     const uint32_t* __restrict__ nonempty, int64_t n_y,
+                                                     int zero_blocks) {
     __shared__ double buf[MQI_SEG_SHORT];
     const int lane = threadIdx.x;
+    // the first zero_blocks workgroups write y = 0 at the elements no row
+    // reaches (bit clear in nonempty), alongside the segments' workgroups
+    if ((int) blockIdx.x < zero_blocks) {
+        for (int64_t v = (int64_t) blockIdx.x * 64 + lane; v < n_y; v += (int64_t) zero_blocks * 64)
+            if (!((nonempty[v >> 5] >> (v & 31)) & 1u)) y[v] = (TY) 0;
+        return;
+    }
     // plain grid-stride walk: a remap giving each XCD a contiguous run of
     // segments measured ~7 % slower on the 5000-spot Dij
-    for (int64_t s = blockIdx.x; s < n_seg; s += gridDim.x) {
+    const int64_t stride = (int64_t) gridDim.x - zero_blocks;
+    for (int64_t s = (int64_t) blockIdx.x - zero_blocks; s < n_seg; s += stride) {
         const uint32_t info = seg_info[s];
         const int64_t  kb = seg_k[s], ke = seg_k[s + 1];
         if (info >> 31) {
@@ -464,6 +475,16 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
             for (int64_t rb = r0; rb < r1; rb += 64 / L) {
                 const int64_t r   = rb + lane / L;
                 double        acc = 0.0;
+                // the row's prescription and weight are loaded with its
+                // pointers, not after its sum (one round trip per row)
+                // row r of the matrix is element v of y (rowmap: the voxel of
+                // each row of a matrix with its empty rows left out)
+                const int64_t v  = (rowmap && r < r1) ? (int64_t) rowmap[r] : r;
+                double        pr = 0.0, cw = 1.0;
+                if (OBJ && r < r1 && sub == 0) {
+                    pr = o.p[v];
+                    if (o.c) cw = (double) o.c[v];
+                }
                 if (r < r1) {
                     const int64_t e = indptr[r + 1] - kb;
                     for (int64_t k = indptr[r] - kb + sub; k < e; k += L) acc += buf[k];
@@ -472,12 +493,11 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
                 if (r < r1 && sub == 0) {
                     const double out = alpha * acc;
                     if (OBJ) {
-                        const double res = out - o.p[r];
-                        const double cw  = o.c ? (double) o.c[r] : 1.0;
+                        const double res = out - pr;
                         obj += cw * res * res;
-                        if (o.g) o.g[r] = (float) (o.two_inv_n * cw * res);
+                        if (o.g) o.g[v] = (float) (o.two_inv_n * cw * res);
                     } else {
-                        y[r] = accumulate ? (TY) ((double) y[r] + out) : (TY) out;
+                        y[v] = accumulate ? (TY) ((double) y[v] + out) : (TY) out;
                     }
                 }
             }
@@ -495,11 +515,12 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
 template<typename TY, bool OBJ>
 __global__ void k_csr_split_rows(const uint32_t* __restrict__ seg_info, const int32_t* __restrict__ split_seg,
                                  int64_t n_split, const double* __restrict__ part, TY* __restrict__ y, double alpha,
-                                 int accumulate, quad_obj o) {
+                                 int accumulate, quad_obj o, const int32_t* __restrict__ rowmap) {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_split) return;
     const int32_t s0 = split_seg[2 * i], s1 = split_seg[2 * i + 1];
-    const int64_t r  = seg_info[s0] & 0x7fffffffu;
+    const int64_t rr = seg_info[s0] & 0x7fffffffu;
+    const int64_t r  = rowmap ? (int64_t) rowmap[rr] : rr;   // the element of y
     double        acc = 0.0;
     for (int32_t s = s0; s < s1; ++s) acc += part[s];
     const double out = alpha * acc;
@@ -528,73 +549,109 @@ template<typename TC>
 int
 csr_adaptive_t(const int64_t* indptr, const TC* col, const float* val, const float* x, const int64_t* seg_k,
                const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-               void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+               void* y, int y_is_double, double alpha, int accumulate, const int32_t* rowmap,
+               const uint32_t* nonempty, int64_t n_y, hipStream_t s) {
+    if (rowmap && !accumulate && (!nonempty || n_y <= 0)) return (int) hipErrorInvalidValue;
     if (n_seg <= 0) return 0;
     const int blocks = (int) (n_seg < 262144 ? n_seg : 262144);
+    const int zb     = (rowmap && !accumulate) ? MQI_ZERO_BLOCKS : 0;
     const quad_obj o{};
     if (y_is_double)
-        hipLaunchKernelGGL((k_csr_adaptive<double, TC, false>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x,
-                           seg_k, seg_info, n_seg, part, (double*) y, alpha, accumulate, o);
+        hipLaunchKernelGGL((k_csr_adaptive<double, TC, false>), dim3(blocks + zb), dim3(64), 0, s, indptr, col, val,
+                           x, seg_k, seg_info, n_seg, part, (double*) y, alpha, accumulate, o, rowmap, nonempty, n_y,
+                           zb);
     else
-        hipLaunchKernelGGL((k_csr_adaptive<float, TC, false>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x,
-                           seg_k, seg_info, n_seg, part, (float*) y, alpha, accumulate, o);
+        hipLaunchKernelGGL((k_csr_adaptive<float, TC, false>), dim3(blocks + zb), dim3(64), 0, s, indptr, col, val,
+                           x, seg_k, seg_info, n_seg, part, (float*) y, alpha, accumulate, o, rowmap, nonempty, n_y,
+                           zb);
     if (n_split > 0) {
         const int b = (int) ((n_split + 255) / 256);
         if (y_is_double)
             hipLaunchKernelGGL((k_csr_split_rows<double, false>), dim3(b), dim3(256), 0, s, seg_info, split_seg,
-                               n_split, part, (double*) y, alpha, accumulate, o);
+                               n_split, part, (double*) y, alpha, accumulate, o, rowmap);
         else
             hipLaunchKernelGGL((k_csr_split_rows<float, false>), dim3(b), dim3(256), 0, s, seg_info, split_seg,
-                               n_split, part, (float*) y, alpha, accumulate, o);
+                               n_split, part, (float*) y, alpha, accumulate, o, rowmap);
     }
     return (int) hipGetLastError();
+}
+
+// the objective's terms of the elements of y no row reaches (bit v of
+// nonempty clear): c p^2 per element, summed per workgroup of a fixed grid
+// into part[0 .. MQI_QUAD_EMPTY_BLOCKS), and g = -two_inv_n c p there
+__global__ void k_quad_empty(const uint32_t* __restrict__ nonempty, int64_t n, quad_obj o, double* __restrict__ part) {
+    double acc = 0.0;
+    for (int64_t v = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t) gridDim.x * blockDim.x) {
+        if ((nonempty[v >> 5] >> (v & 31)) & 1u) continue;
+        const double p  = o.p[v];
+        const double cw = o.c ? (double) o.c[v] : 1.0;
+        acc += cw * p * p;
+        if (o.g) o.g[v] = (float) (-o.two_inv_n * cw * p);
+    }
+    typedef hipcub::BlockReduce<double, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    const double t = BR(tmp).Sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 template<typename TC>
 int
 csr_quadratic_t(const int64_t* indptr, const TC* col, const float* val, const float* x, const int64_t* seg_k,
                 const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-                double alpha, quad_obj o, double inv_n, double* f, hipStream_t s) {
+                double alpha, quad_obj o, double inv_n, double* f, const int32_t* rowmap, const uint32_t* nonempty,
+                int64_t n_y, hipStream_t s) {
     if (n_seg <= 0 || !o.part || !o.p || !f) return (int) hipErrorInvalidValue;
+    if (rowmap && !nonempty) return (int) hipErrorInvalidValue;
     o.n_seg          = n_seg;
     const int blocks = (int) (n_seg < 262144 ? n_seg : 262144);
+    int64_t   n_part = n_seg + n_split;
+    if (rowmap) {   // the elements without a row first (their g is not written again)
+        hipLaunchKernelGGL(k_quad_empty, dim3(MQI_QUAD_EMPTY_BLOCKS), dim3(256), 0, s, nonempty, n_y, o,
+                           o.part + n_part);
+        n_part += MQI_QUAD_EMPTY_BLOCKS;
+    }
     hipLaunchKernelGGL((k_csr_adaptive<double, TC, true>), dim3(blocks), dim3(64), 0, s, indptr, col, val, x, seg_k,
-                       seg_info, n_seg, part, (double*) nullptr, alpha, 0, o);
+                       seg_info, n_seg, part, (double*) nullptr, alpha, 0, o, rowmap, (const uint32_t*) nullptr,
+                       (int64_t) 0, 0);
     if (n_split > 0)
         hipLaunchKernelGGL((k_csr_split_rows<double, true>), dim3((unsigned) ((n_split + 255) / 256)), dim3(256), 0, s,
-                           seg_info, split_seg, n_split, part, (double*) nullptr, alpha, 0, o);
-    hipLaunchKernelGGL(k_quad_sum, dim3(1), dim3(256), 0, s, o.part, n_seg + n_split, inv_n, f);
+                           seg_info, split_seg, n_split, part, (double*) nullptr, alpha, 0, o, rowmap);
+    hipLaunchKernelGGL(k_quad_sum, dim3(1), dim3(256), 0, s, o.part, n_part, inv_n, f);
     return (int) hipGetLastError();
 }
 
 int
 csr_adaptive(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
              const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-             void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+             void* y, int y_is_double, double alpha, int accumulate, const int32_t* rowmap, const uint32_t* nonempty,
+             int64_t n_y, hipStream_t s) {
     return csr_adaptive_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, y, y_is_double,
-                          alpha, accumulate, s);
+                          alpha, accumulate, rowmap, nonempty, n_y, s);
 }
 int
 csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
                const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-               void* y, int y_is_double, double alpha, int accumulate, hipStream_t s) {
+               void* y, int y_is_double, double alpha, int accumulate, const int32_t* rowmap, const uint32_t* nonempty,
+             int64_t n_y, hipStream_t s) {
     return csr_adaptive_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, y, y_is_double,
-                          alpha, accumulate, s);
+                          alpha, accumulate, rowmap, nonempty, n_y, s);
 }
 
 int
 csr_quadratic(const int64_t* indptr, const int32_t* col, const float* val, const float* x, const int64_t* seg_k,
               const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-              double alpha, quad_obj o, double inv_n, double* f, hipStream_t s) {
+              double alpha, quad_obj o, double inv_n, double* f, const int32_t* rowmap, const uint32_t* nonempty,
+              int64_t n_y, hipStream_t s) {
     return csr_quadratic_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, alpha, o, inv_n, f,
-                           s);
+                           rowmap, nonempty, n_y, s);
 }
 int
 csr_quadratic16(const int64_t* indptr, const uint16_t* col, const float* val, const float* x, const int64_t* seg_k,
                 const uint32_t* seg_info, int64_t n_seg, const int32_t* split_seg, int64_t n_split, double* part,
-                double alpha, quad_obj o, double inv_n, double* f, hipStream_t s) {
+                double alpha, quad_obj o, double inv_n, double* f, const int32_t* rowmap, const uint32_t* nonempty,
+                int64_t n_y, hipStream_t s) {
     return csr_quadratic_t(indptr, col, val, x, seg_k, seg_info, n_seg, split_seg, n_split, part, alpha, o, inv_n, f,
-                           s);
+                           rowmap, nonempty, n_y, s);
 }
 
 // ------------------------------------------------------------------ D^T g over column deltas
