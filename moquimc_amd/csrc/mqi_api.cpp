@@ -1568,6 +1568,10 @@ prepare_products(mqi_ctx* c, csr_dev& m, const std::vector<int64_t>& indptr, boo
         // sort every row by voxel (rows handed in by the caller are checked
         // sorted instead)
         if (sort_rows) {
+            // the segmented radix sort counts items in 32 bits
+            if (m.nnz > (int64_t) INT32_MAX)
+                return fail(c, MQI_E_CAPACITY, "%lld nonzeros in one Dij exceed the row sort's 2^31 - 1",
+                            (long long) m.nnz);
             int32_t* tk = nullptr;
             float*   tv = nullptr;
             if ((r = dalloc(c, &tk, m.nnz)) || (r = dalloc(c, &tv, m.nnz))) return r;

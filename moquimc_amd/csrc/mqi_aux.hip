@@ -314,6 +314,7 @@ int
 segmented_sort(int32_t* keys, float* vals, int32_t* keys_tmp, float* vals_tmp, int64_t nnz, int n_seg,
                const int64_t* seg_begin, const int64_t* seg_end, void* scratch, size_t* scratch_bytes,
                hipStream_t s) {
+    if (nnz > (int64_t) INT32_MAX) return (int) hipErrorInvalidValue;   // hipcub counts items in an int
     hipcub::DoubleBuffer<int32_t> k(keys, keys_tmp);
     hipcub::DoubleBuffer<float>   v(vals, vals_tmp);
     hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(scratch, *scratch_bytes, k, v, (int) nnz, n_seg,
