@@ -34,7 +34,8 @@ struct csr_plan {
     uint32_t* seg_info  = nullptr;   // n_seg + 1
     int32_t*  split_seg = nullptr;   // 2 per split row
     double*   part      = nullptr;   // n_seg
-    double*   qpart     = nullptr;   // n_seg + n_split + MQI_QUAD_EMPTY_BLOCKS: objective terms (on first use)
+    double*   qpart     = nullptr;   // n_seg + n_split + MQI_QUAD_EMPTY_BLOCKS + MQI_QUAD_SUM_BLOCKS: objective
+                                     // terms and chunk sums (on first use)
 };
 
 struct csr_dev {
@@ -1988,7 +1989,9 @@ mqi_dij_quadratic(mqi_ctx* c, int s, const float* w, const double* p, const floa
     if (m.nnz == 0) return fail(c, MQI_E_STATE, "scenario %d's Dij has no nonzeros", s);
     csr_plan& P = m.tplan;
     int       r = 0;
-    if (!P.qpart && (r = dalloc(c, &P.qpart, P.n_seg + P.n_split + MQI_QUAD_EMPTY_BLOCKS))) return r;
+    if (!P.qpart &&
+        (r = dalloc(c, &P.qpart, P.n_seg + P.n_split + MQI_QUAD_EMPTY_BLOCKS + MQI_QUAD_SUM_BLOCKS)))
+        return r;
     hipStream_t       st = stream ? (hipStream_t) stream : c->stream;
     mqi_aux::quad_obj o;
     o.p         = p;

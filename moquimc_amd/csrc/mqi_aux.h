@@ -59,7 +59,10 @@ int csr_adaptive16(const int64_t* indptr, const uint16_t* col, const float* val,
 // bit in nonempty is clear (d = 0) get their terms from a fixed grid of
 // MQI_QUAD_EMPTY_BLOCKS workgroups (part then holds n_seg + n_split +
 // MQI_QUAD_EMPTY_BLOCKS doubles).
-#define MQI_QUAD_EMPTY_BLOCKS 1024
+#define MQI_QUAD_EMPTY_BLOCKS 2048
+// the terms are summed in MQI_QUAD_SUM_BLOCKS fixed chunks, then the chunk
+// sums (part holds MQI_QUAD_SUM_BLOCKS more doubles for them)
+#define MQI_QUAD_SUM_BLOCKS 256
 struct quad_obj {
     const double* p;
     const float*  c;

@@ -378,8 +378,18 @@ __global__ void __launch_bounds__(64) k_csr_adaptive(const int64_t* __restrict__
     // the first zero_blocks workgroups write y = 0 at the elements no row
     // reaches (bit clear in nonempty), alongside the segments' workgroups
     if ((int) blockIdx.x < zero_blocks) {
-        for (int64_t v = (int64_t) blockIdx.x * 64 + lane; v < n_y; v += (int64_t) zero_blocks * 64)
-            if (!((nonempty[v >> 5] >> (v & 31)) & 1u)) y[v] = (TY) 0;
+        // eight elements per lane and pass, their mask words loaded together
+        const int64_t zs = (int64_t) zero_blocks * 64;
+        for (int64_t v0 = (int64_t) blockIdx.x * 64 + lane; v0 < n_y; v0 += 8 * zs) {
+            uint32_t bits[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bits[j] = (v0 + j * zs < n_y) ? nonempty[(v0 + j * zs) >> 5] : 0xffffffffu;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t v = v0 + j * zs;
+                if (v < n_y && !((bits[j] >> (v & 31)) & 1u)) y[v] = (TY) 0;
+            }
+        }
         return;
     }
     // plain grid-stride walk: a remap giving each XCD a contiguous run of
@@ -537,6 +547,30 @@ __global__ void k_csr_split_rows(const uint32_t* __restrict__ seg_info, const in
 
 // f = inv_n * (o.part[0] + ... + o.part[n - 1]) in a fixed order (one
 // workgroup: strided partial sums, then a block reduction)
+// Fixed-order sum of the objective's terms in two launches: workgroup b
+// sums chunk b of part (each thread its elements in index order, four loads
+// in flight) into out[b]; a single workgroup then sums the
+// MQI_QUAD_SUM_BLOCKS chunk sums and scales by inv_n.
+__global__ void k_quad_chunks(const double* __restrict__ part, int64_t n, double* __restrict__ out) {
+    const int64_t ch = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t) blockIdx.x * ch;
+    const int64_t b1 = b0 + ch < n ? b0 + ch : n;
+    double        acc = 0.0;
+    int64_t       i   = b0 + threadIdx.x;
+    for (; i + 768 < b1; i += 1024) {
+        const double a0 = part[i], a1 = part[i + 256], a2 = part[i + 512], a3 = part[i + 768];
+        acc += a0;
+        acc += a1;
+        acc += a2;
+        acc += a3;
+    }
+    for (; i < b1; i += 256) acc += part[i];
+    typedef hipcub::BlockReduce<double, 256> BR;
+    __shared__ typename BR::TempStorage tmp;
+    const double t = BR(tmp).Sum(acc);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 __global__ void k_quad_sum(const double* __restrict__ part, int64_t n, double inv_n, double* __restrict__ f) {
     double acc = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += 256) acc += part[i];
@@ -581,13 +615,34 @@ csr_adaptive_t(const int64_t* indptr, const TC* col, const float* val, const flo
 // nonempty clear): c p^2 per element, summed per workgroup of a fixed grid
 // into part[0 .. MQI_QUAD_EMPTY_BLOCKS), and g = -two_inv_n c p there
 __global__ void k_quad_empty(const uint32_t* __restrict__ nonempty, int64_t n, quad_obj o, double* __restrict__ part) {
-    double acc = 0.0;
-    for (int64_t v = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t) gridDim.x * blockDim.x) {
-        if ((nonempty[v >> 5] >> (v & 31)) & 1u) continue;
-        const double p  = o.p[v];
-        const double cw = o.c ? (double) o.c[v] : 1.0;
-        acc += cw * p * p;
-        if (o.g) o.g[v] = (float) (-o.two_inv_n * cw * p);
+    // eight elements per thread and pass, their loads issued together (one
+    // load in flight per thread left the pass latency-bound); the thread's
+    // sum runs over its elements in index order
+    constexpr int U      = 8;
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+    double        acc    = 0.0;
+    for (int64_t v0 = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; v0 < n; v0 += U * stride) {
+        uint32_t bits[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int64_t v = v0 + j * stride;
+            bits[j]         = v < n ? nonempty[v >> 5] : 0xffffffffu;
+        }
+        double p[U], cw[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int64_t v     = v0 + j * stride;
+            const bool    empty = v < n && !((bits[j] >> (v & 31)) & 1u);
+            p[j]                = empty ? o.p[v] : 0.0;
+            cw[j]               = (empty && o.c) ? (double) o.c[v] : 1.0;
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int64_t v = v0 + j * stride;
+            if (v >= n || ((bits[j] >> (v & 31)) & 1u)) continue;
+            acc += cw[j] * p[j] * p[j];
+            if (o.g) o.g[v] = (float) (-o.two_inv_n * cw[j] * p[j]);
+        }
     }
     typedef hipcub::BlockReduce<double, 256> BR;
     __shared__ typename BR::TempStorage tmp;
@@ -617,7 +672,9 @@ csr_quadratic_t(const int64_t* indptr, const TC* col, const float* val, const fl
     if (n_split > 0)
         hipLaunchKernelGGL((k_csr_split_rows<double, true>), dim3((unsigned) ((n_split + 255) / 256)), dim3(256), 0, s,
                            seg_info, split_seg, n_split, part, (double*) nullptr, alpha, 0, o, rowmap);
-    hipLaunchKernelGGL(k_quad_sum, dim3(1), dim3(256), 0, s, o.part, n_part, inv_n, f);
+    double* chunks = o.part + n_seg + n_split + MQI_QUAD_EMPTY_BLOCKS;
+    hipLaunchKernelGGL(k_quad_chunks, dim3(MQI_QUAD_SUM_BLOCKS), dim3(256), 0, s, o.part, n_part, chunks);
+    hipLaunchKernelGGL(k_quad_sum, dim3(1), dim3(256), 0, s, chunks, (int64_t) MQI_QUAD_SUM_BLOCKS, inv_n, f);
     return (int) hipGetLastError();
 }
 
