@@ -79,8 +79,9 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers, atomic_ceiling):
                                                   "device, this run (clustered streams run faster: "
                                                   "profiles/atomic_probes.md)"),
                                   "counter_source": COUNTER_SOURCE}
-    out["limiter"] = ("the FP64 atomic stream of the two scored arrays (atomic_roofline); VALU issue and HBM "
-                      "bandwidth are not at their ceilings (" + COUNTER_SOURCE + ")")
+    out["limiter"] = ("the FP64 atomic requests of the two scored arrays, issued at the device's random-atomic "
+                      "rate (atomic_roofline), with the VALU issue slots near their ceiling (valu_issue_frac); "
+                      "HBM bandwidth is not at its ceiling (" + COUNTER_SOURCE + ")")
     return out
 
 
