@@ -59,7 +59,7 @@ typedef struct mqi_run_config {
     int32_t  stack_depth;      /* secondary stack per lane; 0: the reference's (200 debug, 10 release) */
     int32_t  block_size;       /* threads per workgroup; 0: 256 */
     int32_t  blocks;           /* workgroups of the persistent launch; 0: fill the device */
-    int32_t  waves_per_simd;   /* register budget of the production kernels in waves/SIMD: 2 (compiler's), 3-6, 8; 0: 4 */
+    int32_t  waves_per_simd;   /* register budget of the production kernels in waves/SIMD: 2 (compiler's), 3-6, 8; 0: 3 */
     int32_t  fetch_chunk;      /* histories a lane takes per work-queue fetch (batched); 0: 1 */
     int32_t  refill_lanes;     /* finished lanes a wave gathers before fetching new histories (batched); 0: 4 */
     int32_t  dij_slots_sqrt;   /* > 0: the Dij table of a (scenario, spot) with h histories gets

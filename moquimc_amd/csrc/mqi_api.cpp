@@ -569,7 +569,8 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.item_scenario = c->d_item_scen;
     P.queue         = c->d_queue;
     P.fetch_chunk   = cfg.fetch_chunk > 0 ? cfg.fetch_chunk : 1;
-    // 0: 4, the measured optimum on the 2000-spot CT plan (1 refills at once)
+    // 0: 4 (1 refills at once; whole-wave refills, 64, cluster the CT plan's
+    // atomics at its tuned launch but slow the Dij and slab plans)
     P.refill_lanes  = cfg.refill_lanes > 0 ? cfg.refill_lanes : 4;
     P.tracked       = c->d_counters + 1;
     P.drops         = c->d_counters + 2;

@@ -1176,16 +1176,17 @@ variant pick_waves(int waves) {
     return make<D, Y, T, 0>();
 #else
     if (T) return make<D, Y, T, 0>();
-    // 0: the measured optimum of the production kernels (4 waves/SIMD: the
-    // transport is latency-bound and past 4 the spills cost more than the
-    // extra waves hide); 2: the compiler's own register budget
+    // 0: 3 waves/SIMD, the measured optimum of the production kernels with
+    // deferred scoring on every plan measured (CT, Dij, robust, slab and
+    // water phantoms: 4-16% faster than 4, whose extra waves add atomics in
+    // flight and spills); 2: the compiler's own register budget
     switch (waves) {
     case 2: return make<D, Y, T, 0>();
-    case 3: return make<D, Y, T, 3>();
+    case 4: return make<D, Y, T, 4>();
     case 5: return make<D, Y, T, 5>();
     case 6: return make<D, Y, T, 6>();
     case 8: return make<D, Y, T, 8>();
-    default: return make<D, Y, T, 4>();
+    default: return make<D, Y, T, 3>();
     }
 #endif
 }
