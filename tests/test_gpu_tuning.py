@@ -22,11 +22,32 @@ def _setup(ctx):
 def test_launch_shape_does_not_change_results(ctx):
     _setup(ctx)
     outs = []
-    for block, chunk, blocks in [(256, 1, 0), (128, 4, 0), (64, 2, 7), (192, 1, 3)]:
+    # refill_lanes 64: a wave takes new histories only when all its lanes are done
+    for block, chunk, blocks, refill in [(256, 1, 0, 0), (128, 4, 0, 64), (64, 2, 7, 8), (192, 1, 3, 1),
+                                         (128, 2, 0, 64)]:
         ctx.zero_scorers()
         res = ctx.run(ctx.config(seed=41, batched=True, exact=True, block_size=block, fetch_chunk=chunk,
-                                 blocks=blocks))
+                                 blocks=blocks, refill_lanes=refill))
         assert res.histories == 1500
+        outs.append([ctx.scorer(k) for k in range(3)])
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            fin = np.isfinite(a)
+            assert np.array_equal(fin, np.isfinite(b))
+            assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=0)
+
+
+def test_production_whole_wave_refills_match_default(ctx):
+    """The production kernels with whole-wave refills (the autotuner's pick on
+    the CT plan) score the same histories as with the default refills: equal
+    up to the FP64 summation order of the atomics."""
+    _setup(ctx)
+    outs = []
+    for refill, chunk in [(0, 0), (64, 2), (32, 1)]:
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=47, batched=True, waves_per_simd=3, block_size=128, fetch_chunk=chunk,
+                                 refill_lanes=refill))
+        assert res.histories == 1500 and res.status == 0
         outs.append([ctx.scorer(k) for k in range(3)])
     for o in outs[1:]:
         for a, b in zip(outs[0], o):
