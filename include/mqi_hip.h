@@ -142,7 +142,7 @@ int mqi_run(mqi_ctx* ctx, const mqi_run_config* cfg, mqi_run_result* res);
  * block and one history per thread, mqi_tps_env.hpp:1058-1083): times
  * batched passes of the current plan with every spot's histories scaled by
  * `sample` (0 < sample <= 1): waves_per_simd {3, 4, 5} x block_size {128, 256},
- * then refill_lanes {1, 2, 4, 8, 16, 32, 64}, then fetch_chunk {1, 2, 4} (coordinate
+ * then refill_lanes {1, 2, 4, 8, 16, 32, 60, 62, 64}, then fetch_chunk {1, 2, 4} (coordinate
  * descent), and writes the fastest into *cfg.  With
  * cfg->stack_depth < 0 it also picks the smallest secondary stack (16 .. 512)
  * that drops nothing on the sample (the reference's depth drops silently,

@@ -1305,11 +1305,11 @@ mqi_autotune(mqi_ctx* c, mqi_run_config* cfg, double sample, double* best_ms) {
             t.block_size     = b;
             if ((r = try_cfg(t))) return r;
         }
-    // (a wave refilled as a whole keeps its lanes at similar depths, so
-    // their scoring atomics cluster: 5.7% on the 2000-spot CT plan at 3
-    // waves/SIMD, slower on the Dij and slab plans)
+    // (a wave refilled as a whole, or nearly, keeps its lanes at similar
+    // depths, so their scoring atomics cluster: 5.7-6.9% on the 2000-spot
+    // CT plan at 3 waves/SIMD, slower on the Dij and slab plans)
     const mqi_run_config stage2 = pick;
-    for (int k : {1, 2, 8, 16, 32, 64}) {
+    for (int k : {1, 2, 8, 16, 32, 60, 62, 64}) {
         mqi_run_config t = stage2;
         t.refill_lanes   = k;
         if ((r = try_cfg(t))) return r;
