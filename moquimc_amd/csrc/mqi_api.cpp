@@ -89,6 +89,12 @@ struct mqi_ctx {
     std::vector<mqi_spot> spots;
     mqi_spot*             d_spots = nullptr;
     int                   n_scen  = 1;
+    // paired scoring: the interleaved array of scorers 0 and 1 (2 doubles
+    // per element, zero between passes) and the arrays it folds into
+    double*               d_pair     = nullptr;
+    size_t                pair_cap   = 0;   // elements
+    size_t                pair_n     = 0;   // elements of the current pass (0: not paired)
+    double*               pair_real[2] = {nullptr, nullptr};
     std::vector<double>   scen_budget;   // histories factor per scenario (empty: 1)
     bool                  scen_set = false;   // mqi_set_scenarios gave explicit shifts / scales
     float *               d_shift = nullptr, *d_scale = nullptr;
@@ -644,6 +650,40 @@ launch_bound(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dyn
                                       c->stream);
 }
 
+// Paired scoring for a batched pass of the production kernels with each
+// scorer scored once, no Dij and at least two scorers: scorers 0 and 1 add
+// into one interleaved array (both sums of a voxel in one line, added by one
+// atomic instruction from a pair of lanes), folded into their arrays after
+// the pass.
+int
+pair_setup(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
+    c->pair_n = 0;
+    if (cfg.exact || cfg.dij || cfg.scoring_loops != MQI_LOOPS_ONCE || P.n_scorers < 2 ||
+        P.kinds[0] == MQI_KIND_DIJ || P.kinds[1] == MQI_KIND_DIJ)
+        return 0;
+    const size_t n = (size_t) nvox(c) * (c->per_scen ? (size_t) c->n_scen : 1u);
+    if (n > c->pair_cap) {
+        dfree(c->d_pair);
+        c->pair_cap = 0;
+        int r = dalloc(c, &c->d_pair, 2 * n);
+        if (r) return r;
+        KCHK(c, mqi_aux::fill_u64(c->d_pair, 0ull, 2 * n, c->stream));
+        c->pair_cap = n;
+    }
+    c->pair_real[0] = P.out[0];
+    c->pair_real[1] = P.out[1];
+    c->pair_n       = n;
+    P.pair          = 1;
+    P.pair_out      = c->d_pair;
+    return 0;
+}
+
+int
+pair_fold(mqi_ctx* c) {
+    if (c->pair_n) KCHK(c, mqi_aux::pair_fold(c->pair_real[0], c->pair_real[1], c->d_pair, c->pair_n, c->stream));
+    return 0;
+}
+
 int
 launch(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dynamic, int trace, int blocks,
        int threads) {
@@ -726,6 +766,7 @@ graph_pass(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int block
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if (int e = pair_fold(c)) return e;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float t = 0;
@@ -756,9 +797,11 @@ transport_pass(mqi_ctx* c, const mqi_run_config& cfg, uint64_t hoff, double* ms)
         const int blocks = persistent_blocks(c, cfg, threads, 0);
         if ((r = ensure_stack(c, lane_bytes * (size_t) blocks * threads))) return r;
         P.stack = c->d_stack;
+        if ((r = pair_setup(c, P, cfg))) return r;
         if (cfg.use_graph) return graph_pass(c, P, cfg, blocks, threads, ms);
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         KCHK(c, launch(c, P, cfg, 1, 0, blocks, threads));
+        if ((r = pair_fold(c))) return r;
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev1));
         float t = 0;
@@ -976,6 +1019,7 @@ mqi_destroy(mqi_ctx* c) {
     dfree(c->d_statsum);
     dfree(c->d_statcnt);
     dfree(c->d_stack);
+    dfree(c->d_pair);
     dfree(c->d_item_end);
     dfree(c->d_item_spot);
     dfree(c->d_item_scen);

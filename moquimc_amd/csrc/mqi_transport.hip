@@ -543,12 +543,55 @@ MQI_DI void flush(const transport_params& P, pending& A, double* v) {
             if (x == 0.0) continue;
             if (P.kinds[s] == MQI_KIND_DIJ)
                 dij_insert(P, A.item, (uint32_t) A.cnb, x);
+#if !MQI_EXACT
+            else if (P.pair && s < 2)
+                atomicAdd(P.pair_out + 2 * (A.off + (uint64_t) A.cnb) + s, x);
+#endif
             else
                 atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, x);
         }
     }
     A.cnb = -1;
 }
+
+#if !MQI_EXACT
+// The flush of the step with paired scoring, run by every lane that steps
+// (f: this lane leaves its voxel).  Lanes 2m and 2m+1 exchange their sums:
+// the first atomic instruction adds both of lane 2m's sums (scorer 0 from
+// lane 2m, scorer 1 from lane 2m+1), the second both of lane 2m+1's, so the
+// two adds of a voxel land in one line of the interleaved array in one
+// instruction.  A lane whose partner does not step adds both of its own.
+MQI_DI void flush_pair(const transport_params& P, pending& A, double* v, bool f) {
+    const bool     has = f && A.cnb >= 0;
+    const double   a   = has ? v[0] : 0.0;
+    const double   b   = has ? v[blockDim.x] : 0.0;
+    const uint64_t k   = has ? A.off + (uint64_t) A.cnb : 0;
+    const int      me  = __lane_id();
+    const bool     pact = (__ballot(1) >> (me ^ 1)) & 1ull;
+    const uint64_t kp  = __shfl_xor((unsigned long long) k, 1);
+    const double   ap  = __shfl_xor(a, 1);
+    const double   bp  = __shfl_xor(b, 1);
+    const bool     odd = me & 1;
+    uint64_t       eA = 2 * k, eB = 2 * k + 1;   // partner stepping: overwritten
+    double         xA = a, xB = b;
+    if (pact) {
+        eA = odd ? 2 * kp + 1 : 2 * k;
+        xA = odd ? bp : a;
+        eB = odd ? 2 * k + 1 : 2 * kp;
+        xB = odd ? b : ap;
+    }
+    if (xA != 0.0) atomicAdd(P.pair_out + eA, xA);
+    if (xB != 0.0) atomicAdd(P.pair_out + eB, xB);
+    if (has) {
+#pragma unroll
+        for (int s = 2; s < MQI_DEFER; ++s) {
+            const double x = v[s * blockDim.x];
+            if (x != 0.0) atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, x);
+        }
+        A.cnb = -1;
+    }
+}
+#endif
 
 template<bool DBG>
 MQI_DI void score(const transport_params& P, const trk& t, uint64_t cnb, float density, float volume, float sp,
@@ -884,8 +927,14 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb, d
         // step: one counter tracks loads, stores and atomics alike, so atomics
         // issued just before the next wait on a load would add their round
         // trip to every step.
-        if (P.loops == MQI_LOOPS_ONCE &&
-            ((int64_t) cnb != L.acc.cnb || L.scen_off != L.acc.off || (uint32_t) L.dij_item != L.acc.item))
+        const bool leave = P.loops == MQI_LOOPS_ONCE && ((int64_t) cnb != L.acc.cnb || L.scen_off != L.acc.off ||
+                                                         (uint32_t) L.dij_item != L.acc.item);
+#if !MQI_EXACT
+        if (P.pair)
+            flush_pair(P, L.acc, pv, leave);
+        else
+#endif
+        if (leave)
             flush(P, L.acc, pv);
     }
     float sp;   // stopping power ratio of the step

@@ -56,6 +56,31 @@ def test_production_whole_wave_refills_match_default(ctx):
             assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=0)
 
 
+def test_paired_scoring_matches_the_plain_launch(ctx):
+    """Batched production passes add scorers 0 and 1 into one interleaved
+    array from pairs of lanes (folded after the pass); the plain per-spot
+    launches score each array directly.  Both transport the same histories,
+    so every scorer agrees up to the FP64 summation order: dose + LETd (slot
+    2 outside the pair) and dose + the statistics pair (Dose derived from
+    Dose_stat)."""
+    _setup(ctx)
+    for mask in (rb.SC_DOSE | rb.SC_LETD, rb.SC_DOSE | rb.SC_STAT):
+        ctx.set_scorers(mask)
+        outs = []
+        for batched, refill in ((True, 8), (True, 64), (False, 0)):
+            ctx.zero_scorers()
+            res = ctx.run(ctx.config(seed=48, batched=batched, waves_per_simd=3, refill_lanes=refill))
+            assert res.histories == 1500 and res.status == 0
+            outs.append([ctx.scorer(k) for k in range(ctx.scorer_count())])
+        assert len(outs[0]) == 3
+        for o in outs[1:]:
+            for a, b in zip(outs[0], o):
+                fin = np.isfinite(a)
+                assert np.array_equal(fin, np.isfinite(b))
+                assert np.any(a[fin] != 0)
+                assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=0)
+
+
 def test_production_occupancy_variants_run(ctx):
     _setup(ctx)
     ref = None
