@@ -1314,6 +1314,17 @@ mqi_autotune(mqi_ctx* c, mqi_run_config* cfg, double sample, double* best_ms) {
     } guard{c, c->spots};
     for (auto& sp : c->spots)
         if (sp.histories) sp.histories = std::max<uint64_t>(1, (uint64_t) std::llround((double) sp.histories * sample));
+    // the trials score what mqi_run scores: with Dose derived from Dose_stat
+    // (each scorer once, the statistics pair requested, no ROI) the kernel
+    // scores two arrays, not three
+    int        s1 = -1, s2 = -1;
+    const bool alias = base.scoring_loops == MQI_LOOPS_ONCE && c->kinds[0] == MQI_KIND_DOSE &&
+                       stat_index(c, &s1, &s2) == 0 && !c->roi_set[0] && !c->roi_set[1];
+    struct alias_guard {
+        mqi_ctx* c;
+        ~alias_guard() { c->alias_dose = false; }
+    } ag{c};
+    c->alias_dose = alias;
     double             ms = 0;
     unsigned long long dropped = 0;
     if ((r = trial_pass(c, base, &ms, &dropped))) return r;   // warm-up
