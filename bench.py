@@ -295,7 +295,7 @@ def phantom_bench(ctx_cls, reps=3):
     return {"phantoms": out}
 
 
-def transport_pass(waves=3, block=128, fetch=2, refill=64):
+def transport_pass(waves=3, block=128, fetch=1, refill=4):
     """One production batched pass of the 2000-spot CT plan (dose + the
     statistics pair) with a fixed launch shape: a single k_transport dispatch
     for the counter runs."""
