@@ -79,9 +79,9 @@ def transport_roofline(steps, kernel_ms, launches, n_scorers, atomic_ceiling):
                                                   "device, this run (clustered streams run faster: "
                                                   "profiles/atomic_probes.md)"),
                                   "counter_source": COUNTER_SOURCE}
-    out["limiter"] = ("the FP64 atomic requests of the two scored arrays, issued at the device's random-atomic "
-                      "rate (atomic_roofline), with the VALU issue slots near their ceiling (valu_issue_frac); "
-                      "HBM bandwidth is not at its ceiling (" + COUNTER_SOURCE + ")")
+    out["limiter"] = ("VALU issue (valu_issue_frac of the issue slots busy); the FP64 atomic requests of the "
+                      "paired scorers run below the device's random-atomic rate (atomic_roofline) and HBM "
+                      "bandwidth is far from its ceiling (" + COUNTER_SOURCE + ")")
     return out
 
 
