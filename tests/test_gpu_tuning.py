@@ -81,6 +81,30 @@ def test_paired_scoring_matches_the_plain_launch(ctx):
                 assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=0)
 
 
+def test_paired_scoring_with_per_scenario_arrays(ctx):
+    """Paired scoring over scenarios with their own arrays (element offsets of
+    one scenario per voxel grid): each scenario's batched scorers equal the
+    plain launch's."""
+    _setup(ctx)
+    ctx.set_scenarios(shifts=[[0, 0, 0], [3, 0, 0], [0, -2, 1]], scales=[1.0, 1.02, 0.97])
+    ctx.set_scorers(rb.SC_DOSE | rb.SC_LETD, per_scenario=True)
+    outs = []
+    for batched in (True, False):
+        ctx.zero_scorers()
+        res = ctx.run(ctx.config(seed=49, batched=batched, waves_per_simd=3))
+        assert res.histories == 3 * 1500 and res.status == 0
+        outs.append([[ctx.scorer(k, scenario=sc) for k in range(3)] for sc in range(3)])
+    for sc in range(3):
+        for a, b in zip(outs[0][sc], outs[1][sc]):
+            fin = np.isfinite(a)
+            assert np.array_equal(fin, np.isfinite(b))
+            assert np.any(a[fin] != 0)
+            assert np.allclose(a[fin], b[fin], rtol=1e-9, atol=0)
+    # the scenarios differ (shifted and scaled copies of the plan)
+    assert not np.allclose(outs[0][0][0], outs[0][1][0])
+    ctx.set_scenarios()
+
+
 def test_production_occupancy_variants_run(ctx):
     _setup(ctx)
     ref = None
