@@ -575,8 +575,8 @@ fill_params(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     P.item_scenario = c->d_item_scen;
     P.queue         = c->d_queue;
     P.fetch_chunk   = cfg.fetch_chunk > 0 ? cfg.fetch_chunk : 1;
-    // 0: 4 (1 refills at once; whole-wave refills, 64, cluster the CT plan's
-    // atomics at its tuned launch but slow the Dij and slab plans)
+    // 0: 4 (1 refills at once), the CT plan's optimum with paired scoring;
+    // whole-wave refills (64) cluster the atomics of unpaired passes
     P.refill_lanes  = cfg.refill_lanes > 0 ? cfg.refill_lanes : 4;
     P.tracked       = c->d_counters + 1;
     P.drops         = c->d_counters + 2;
@@ -1361,8 +1361,8 @@ mqi_autotune(mqi_ctx* c, mqi_run_config* cfg, double sample, double* best_ms) {
             if ((r = try_cfg(t))) return r;
         }
     // (a wave refilled as a whole, or nearly, keeps its lanes at similar
-    // depths, so their scoring atomics cluster: 5.7-6.9% on the 2000-spot
-    // CT plan at 3 waves/SIMD, slower on the Dij and slab plans)
+    // depths, so their scoring atomics cluster: 5.7-6.9% on the unpaired
+    // 2000-spot CT plan; with paired scoring its idle lanes cost more)
     const mqi_run_config stage2 = pick;
     for (int k : {1, 2, 8, 16, 32, 60, 62, 64}) {
         mqi_run_config t = stage2;
