@@ -89,12 +89,13 @@ struct mqi_ctx {
     std::vector<mqi_spot> spots;
     mqi_spot*             d_spots = nullptr;
     int                   n_scen  = 1;
-    // paired scoring: the interleaved array of scorers 0 and 1 (2 doubles
-    // per element, zero between passes) and the arrays it folds into
+    // grouped scoring: the interleaved array of the leading scorers (2 or 4
+    // doubles per element, zero between passes) and the arrays it folds into
     double*               d_pair     = nullptr;
-    size_t                pair_cap   = 0;   // elements
-    size_t                pair_n     = 0;   // elements of the current pass (0: not paired)
-    double*               pair_real[2] = {nullptr, nullptr};
+    size_t                pair_cap   = 0;   // doubles
+    size_t                pair_n     = 0;   // elements of the current pass (0: not grouped)
+    int                   pair_ns = 0, pair_stride = 0;
+    double*               pair_real[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<double>   scen_budget;   // histories factor per scenario (empty: 1)
     bool                  scen_set = false;   // mqi_set_scenarios gave explicit shifts / scales
     float *               d_shift = nullptr, *d_scale = nullptr;
@@ -650,37 +651,43 @@ launch_bound(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg, int dyn
                                       c->stream);
 }
 
-// Paired scoring for a batched pass of the production kernels with each
-// scorer scored once, no Dij and at least two scorers: scorers 0 and 1 add
-// into one interleaved array (both sums of a voxel in one line, added by one
-// atomic instruction from a pair of lanes), folded into their arrays after
-// the pass.
+// Grouped scoring for a batched pass of the production kernels with each
+// scorer scored once, no Dij and at least two scorers: the leading deferred
+// scorers (two: pairs of lanes; three or four: groups of four lanes) add
+// into one interleaved array, all sums of a voxel in one line and added by
+// one atomic instruction from the group, folded into their arrays after the
+// pass.
 int
 pair_setup(mqi_ctx* c, transport_params& P, const mqi_run_config& cfg) {
     c->pair_n = 0;
-    if (cfg.exact || cfg.dij || cfg.scoring_loops != MQI_LOOPS_ONCE || P.n_scorers < 2 ||
-        P.kinds[0] == MQI_KIND_DIJ || P.kinds[1] == MQI_KIND_DIJ)
-        return 0;
-    const size_t n = (size_t) nvox(c) * (c->per_scen ? (size_t) c->n_scen : 1u);
-    if (n > c->pair_cap) {
+    if (cfg.exact || cfg.dij || cfg.scoring_loops != MQI_LOOPS_ONCE) return 0;
+    int ns = 0;
+    while (ns < P.n_scorers && ns < MQI_DEFER && P.kinds[ns] != MQI_KIND_DIJ) ++ns;
+    if (ns < 2) return 0;
+    const int    stride = ns == 2 ? 2 : 4;
+    const size_t n      = (size_t) nvox(c) * (c->per_scen ? (size_t) c->n_scen : 1u);
+    if (stride * n > c->pair_cap) {
         dfree(c->d_pair);
         c->pair_cap = 0;
-        int r = dalloc(c, &c->d_pair, 2 * n);
+        int r = dalloc(c, &c->d_pair, stride * n);
         if (r) return r;
-        KCHK(c, mqi_aux::fill_u64(c->d_pair, 0ull, 2 * n, c->stream));
-        c->pair_cap = n;
+        KCHK(c, mqi_aux::fill_u64(c->d_pair, 0ull, stride * n, c->stream));
+        c->pair_cap = stride * n;
     }
-    c->pair_real[0] = P.out[0];
-    c->pair_real[1] = P.out[1];
-    c->pair_n       = n;
-    P.pair          = 1;
-    P.pair_out      = c->d_pair;
+    for (int s = 0; s < ns; ++s) c->pair_real[s] = P.out[s];
+    c->pair_n      = n;
+    c->pair_ns     = ns;
+    c->pair_stride = stride;
+    P.pair         = stride;
+    P.group_ns     = ns;
+    P.pair_out     = c->d_pair;
     return 0;
 }
 
 int
 pair_fold(mqi_ctx* c) {
-    if (c->pair_n) KCHK(c, mqi_aux::pair_fold(c->pair_real[0], c->pair_real[1], c->d_pair, c->pair_n, c->stream));
+    if (c->pair_n)
+        KCHK(c, mqi_aux::pair_fold(c->pair_real, c->pair_ns, c->pair_stride, c->d_pair, c->pair_n, c->stream));
     return 0;
 }
 

@@ -123,9 +123,9 @@ int csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStrea
 int csr_transpose_sorted(const int64_t* indptr, const int32_t* col, const float* val, int64_t n_rows, int64_t nnz,
                          int key_bits, int32_t* keys_tmp, uint64_t* packed, uint64_t* packed_tmp, int32_t* tcol,
                          float* tval, void* scratch, size_t* scratch_bytes, hipStream_t s);
-// paired scoring: out0[i] += pair[2 i], out1[i] += pair[2 i + 1] (i < n), pair
-// cleared
-int pair_fold(double* out0, double* out1, double* pair, uint64_t n, hipStream_t s);
+// grouped scoring: out[s][i] += pair[stride i + s] for s < ns, i < n (stride
+// 2 or 4), pair cleared
+int pair_fold(double* const* out, int ns, int stride, double* pair, uint64_t n, hipStream_t s);
 }   // namespace mqi_aux
 
 #endif

@@ -1095,25 +1095,32 @@ csr_transpose_count(const int32_t* col, int64_t nnz, uint32_t* cnt, hipStream_t 
 
 namespace mqi_aux
 {
-// out0[i] += pair[2 i], out1[i] += pair[2 i + 1] over n elements; the pair
-// array is cleared where it held a sum
-__global__ void k_pair_fold(double* __restrict__ out0, double* __restrict__ out1, double* __restrict__ pair,
-                            uint64_t n) {
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    d2* p2 = reinterpret_cast<d2*>(pair);
+// out[s][i] += pair[stride i + s] for s < ns over n elements (stride 2 or 4);
+// the pair array is cleared where it held a sum
+struct fold_outs {
+    double* o[4];
+};
+__global__ void k_pair_fold(fold_outs out, int ns, int stride, double* __restrict__ pair, uint64_t n) {
     for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t) gridDim.x * blockDim.x) {
-        const d2 x = p2[i];
-        if (x.x != 0.0 || x.y != 0.0) {
-            if (x.x != 0.0) out0[i] += x.x;
-            if (x.y != 0.0) out1[i] += x.y;
-            p2[i] = d2{0.0, 0.0};
+        double* e = pair + (uint64_t) stride * i;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s >= ns) break;
+            const double x = e[s];
+            if (x != 0.0) {
+                out.o[s][i] += x;
+                e[s] = 0.0;
+            }
         }
     }
 }
 int
-pair_fold(double* out0, double* out1, double* pair, uint64_t n, hipStream_t s) {
+pair_fold(double* const* out, int ns, int stride, double* pair, uint64_t n, hipStream_t s) {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_pair_fold, dim3(grid_for(n)), dim3(256), 0, s, out0, out1, pair, n);
+    if (ns < 1 || ns > 4 || ns > stride) return (int) hipErrorInvalidValue;
+    fold_outs o{};
+    for (int k = 0; k < ns; ++k) o.o[k] = out[k];
+    hipLaunchKernelGGL(k_pair_fold, dim3(grid_for(n)), dim3(256), 0, s, o, ns, stride, pair, n);
     return (int) hipGetLastError();
 }
 }   // namespace mqi_aux

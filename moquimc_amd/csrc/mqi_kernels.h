@@ -99,11 +99,13 @@ struct transport_params {
     double*  out[MQI_MAX_SCORERS];
     uint64_t out_scenario_stride;   // 0: scenarios share the arrays; else per-scenario offset (elements)
     int      loops;
-    // Paired scoring (production kernels, each-scorer-once, no Dij): scorers
-    // 0 and 1 add into one interleaved array, element i of scorer s at
-    // pair_out[2 i + s], folded into their own arrays after the pass
+    // Grouped scoring (production kernels, each-scorer-once, no Dij): the
+    // leading group_ns scorers (2 .. 4) add into one interleaved array with
+    // pair (2 or 4; 0: off) doubles per element, element i of scorer s at
+    // pair_out[pair * i + s], folded into their own arrays after the pass
     double*  pair_out;
     int      pair;
+    int      group_ns;
     uint32_t* dij_keys;
     double*   dij_vals;
     const uint64_t* dij_off;  // item i owns slots [dij_off[i], dij_off[i + 1]) of dij_keys/dij_vals

@@ -544,8 +544,8 @@ MQI_DI void flush(const transport_params& P, pending& A, double* v) {
             if (P.kinds[s] == MQI_KIND_DIJ)
                 dij_insert(P, A.item, (uint32_t) A.cnb, x);
 #if !MQI_EXACT
-            else if (P.pair && s < 2)
-                atomicAdd(P.pair_out + 2 * (A.off + (uint64_t) A.cnb) + s, x);
+            else if (P.pair && s < P.group_ns)
+                atomicAdd(P.pair_out + (uint64_t) P.pair * (A.off + (uint64_t) A.cnb) + s, x);
 #endif
             else
                 atomicAdd(P.out[s] + A.off + (uint64_t) A.cnb, x);
@@ -590,6 +590,41 @@ MQI_DI void flush_pair(const transport_params& P, pending& A, double* v, bool f)
         }
         A.cnb = -1;
     }
+}
+
+// The same for three or four grouped scorers (interleaved four doubles per
+// element): in a group of four stepping lanes, instruction j adds the sums of
+// lane 4m+j's voxel, lane 4m+q adding that lane's sum of scorer q, so all of
+// a voxel's sums go out as one line request.  Lanes of a group with a lane
+// that does not step add their own sums.
+MQI_DI void flush_quad(const transport_params& P, pending& A, double* v, bool f) {
+    const bool     has = f && A.cnb >= 0;
+    const double   x0  = has ? v[0] : 0.0;
+    const double   x1  = has ? v[blockDim.x] : 0.0;
+    const double   x2  = has ? v[2 * blockDim.x] : 0.0;
+    const double   x3  = (has && P.group_ns > 3) ? v[3 * blockDim.x] : 0.0;
+    const uint64_t k   = has ? A.off + (uint64_t) A.cnb : 0;
+    const int      me  = __lane_id();
+    const int      q   = me & 3;
+    const int      g0  = me & ~3;
+    if (((__ballot(1) >> g0) & 0xfull) == 0xfull) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t kj = __shfl((unsigned long long) k, g0 + j);
+            const double   y0 = __shfl(x0, g0 + j);
+            const double   y1 = __shfl(x1, g0 + j);
+            const double   y2 = __shfl(x2, g0 + j);
+            const double   y3 = __shfl(x3, g0 + j);
+            const double   y  = q == 0 ? y0 : (q == 1 ? y1 : (q == 2 ? y2 : y3));
+            if (y != 0.0) atomicAdd(P.pair_out + 4 * kj + q, y);
+        }
+    } else if (has) {
+        if (x0 != 0.0) atomicAdd(P.pair_out + 4 * k, x0);
+        if (x1 != 0.0) atomicAdd(P.pair_out + 4 * k + 1, x1);
+        if (x2 != 0.0) atomicAdd(P.pair_out + 4 * k + 2, x2);
+        if (x3 != 0.0) atomicAdd(P.pair_out + 4 * k + 3, x3);
+    }
+    if (has) A.cnb = -1;
 }
 #endif
 
@@ -930,8 +965,10 @@ MQI_DI void step(const transport_params& P, ctx& c, lane& L, const float* shb, d
         const bool leave = P.loops == MQI_LOOPS_ONCE && ((int64_t) cnb != L.acc.cnb || L.scen_off != L.acc.off ||
                                                          (uint32_t) L.dij_item != L.acc.item);
 #if !MQI_EXACT
-        if (P.pair)
+        if (P.pair == 2)
             flush_pair(P, L.acc, pv, leave);
+        else if (P.pair == 4)
+            flush_quad(P, L.acc, pv, leave);
         else
 #endif
         if (leave)
